@@ -1,0 +1,806 @@
+"""HIP emitter: mechanism -> CDNA4 kernels + C-ABI library source.
+
+The ``hip`` language plugin of the generator.  It takes the parsed
+mechanism in the reference's data model (``SpecInfo``/``ReacInfo`` with
+integer species indices after the bath-gas reordering, exactly what the
+reference's emitters receive at create_rate_subs.py:531-552) and writes one
+self-contained ``crs_<name>.hip``: batched kernels for the four functions of
+the wdot path, a fused kernel per precision mode, and the ``extern "C"``
+entry points of ``include/crs_hip.h``.
+
+Execution model: one thermochemical state per lane, wave64, 256 threads per
+block, states contiguous in every structure-of-arrays row so that each
+load/store of a wave is one coalesced 512-byte access.  Species indices are
+compile-time constants (as in the reference's unrolled code), so per-state
+concentrations and species-rate accumulators live in registers; numeric
+mechanism constants live in one ``__constant__`` table read with
+wave-uniform scalar loads.
+
+Numerics of the FP64 kernels follow the reference's emitted expressions
+(rate_subs.py:20-135, 226-704, 707-1073, 1076-1235, 1292-1391) term for
+term, with every constant equal to the value of the literal the reference
+would print (``utils.emitted``).  The stage kernels for eval_conc and
+eval_spec_rates disable FMA contraction so that they round exactly as the
+C subroutines do.
+
+Mixed precision (``CRS_MODE_MIXED``) evaluates Arrhenius, falloff and
+third-body terms in FP32 and keeps equilibrium constants, the net-rate
+subtraction and the species sums in FP64.
+"""
+
+import math
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+from . import chem
+from .utils import (emitted, fmt_e16, fmt_e8, fmt_g6, fmt_w3, get_nu,
+                    is_integer, py2_str)
+
+__all__ = ['MechIR', 'build_ir', 'write_hip', 'emit_hip_source']
+
+
+# ---------------------------------------------------------------------------
+# Mechanism tables ("IR"): every number the reference would print, as the
+# value its literal denotes.
+# ---------------------------------------------------------------------------
+
+@dataclass
+class ArrIR:
+    """Rate-constant expression, rate_subs.py:20-135.
+
+    value = pre * T^nT                     if not has_exp
+          = [pre *] exp([logA] [+ b logT] [- E/T])   otherwise
+    """
+    pre: Optional[float]
+    nT: int
+    has_exp: bool
+    logA: Optional[float]
+    b: Optional[float]
+    E: Optional[float]
+    # raw (unrounded) parameters, for the mixed path's own evaluation
+    A_raw: float = 0.0
+    b_raw: float = 0.0
+    E_raw: float = 0.0
+
+
+def arr_ir(A, b, E):
+    """Build the rate-constant expression the reference would emit."""
+    if A > 0:
+        logA = emitted(fmt_e16(math.log(A)))
+        if not E:
+            if not b:
+                return ArrIR(emitted(py2_str(A)), 0, False, None, None, None,
+                             A, b, E)
+            if isinstance(b, int):
+                return ArrIR(emitted(py2_str(A)), max(b, 0), False, None,
+                             None, None, A, b, E)
+            return ArrIR(None, 0, True, logA, emitted(py2_str(b)), None,
+                         A, b, E)
+        Ek = emitted(fmt_e16(E))
+        if not b:
+            return ArrIR(None, 0, True, logA, None, Ek, A, b, E)
+        return ArrIR(None, 0, True, logA, emitted(py2_str(b)), Ek, A, b, E)
+    if A < 0:
+        if not E:
+            if not b:
+                return ArrIR(emitted(py2_str(A)), 0, False, None, None, None,
+                             A, b, E)
+            if is_integer(b):
+                return ArrIR(emitted(py2_str(A)), max(int(b), 0), False,
+                             None, None, None, A, b, E)
+            return ArrIR(emitted(fmt_e16(A)), 0, True, None,
+                         emitted(py2_str(b)), None, A, b, E)
+        Ek = emitted(fmt_e16(E))
+        if not b:
+            return ArrIR(emitted(fmt_e16(A)), 0, True, None, None, Ek, A, b, E)
+        return ArrIR(emitted(fmt_e16(A)), 0, True, None, emitted(py2_str(b)),
+                     Ek, A, b, E)
+    raise NotImplementedError('zero pre-exponential factor')
+
+
+@dataclass
+class RxnIR:
+    index: int
+    reac: List[Tuple[int, float]]
+    prod: List[Tuple[int, float]]
+    arr: ArrIR
+    rev: bool = False
+    rev_index: Optional[int] = None
+    rev_arr: Optional[ArrIR] = None          # explicit REV parameters
+    kc_groups: List[Tuple[float, list, list]] = field(default_factory=list)
+    kc_pref: float = 1.0
+    kc_dnu: float = 0.0
+    pd_index: Optional[int] = None
+    thd: bool = False                        # plain third-body
+    falloff: Optional[str] = None            # 'troe' | 'sri' | 'lind'
+    eff: List[Tuple[int, float]] = field(default_factory=list)  # (k, eff-1)
+    pdep_sp: Optional[int] = None
+    low: Optional[ArrIR] = None
+    high: Optional[ArrIR] = None
+    falloff_low: bool = False                # LOW given (unimolecular)
+    falloff_high: bool = False               # HIGH given (chem. activated)
+    troe: Optional[dict] = None
+    sri: Optional[dict] = None
+    spec: List[Tuple[int, float]] = field(default_factory=list)  # (k, nu)
+
+
+@dataclass
+class MechIR:
+    name: str
+    nsp: int
+    nr: int
+    nrev: int
+    npd: int
+    inv_mw: List[float]
+    rxns: List[RxnIR]
+    species_names: List[str]
+
+
+def _eff_terms(rxn):
+    """(species, signed eff-1) as the reference prints them (:901-915)."""
+    out = []
+    for isp, eff in rxn.thd_body_eff:
+        if eff == 1.0:
+            continue
+        if eff > 1.0:
+            out.append((isp, emitted(py2_str(eff - 1.0))))
+        else:
+            out.append((isp, -emitted(py2_str(1.0 - eff))))
+    return out
+
+
+def _kc_ir(rxn, specs):
+    """Lumped NASA-7 Kc coefficients (rate_subs.py:410-431, 513-662)."""
+    groups = {}
+    order = []
+    sum_nu = 0
+
+    def add(sp, scale):
+        def one(a):
+            raw = [a[6] - a[0], a[0] - 1.0, a[1] / 2.0, a[2] / 6.0,
+                   a[3] / 12.0, a[4] / 20.0, a[5]]
+            return [x * scale for x in raw]
+        lo, hi = one(sp.lo), one(sp.hi)
+        key = sp.Trange[1]
+        if key in groups:
+            plo, phi = groups[key]
+            lo = [lo[i] + plo[i] for i in range(7)]
+            hi = [hi[i] + phi[i] for i in range(7)]
+        else:
+            order.append(key)
+        groups[key] = (lo, hi)
+
+    for i, isp in enumerate(rxn.prod):
+        if isp in rxn.reac:
+            nu = rxn.prod_nu[i] - rxn.reac_nu[rxn.reac.index(isp)]
+        else:
+            nu = rxn.prod_nu[i]
+        if nu == 0:
+            continue
+        sum_nu += nu
+        add(specs[isp], nu * 1.0)
+    for i, isp in enumerate(rxn.reac):
+        if isp in rxn.prod:
+            continue
+        nu = rxn.reac_nu[i]
+        sum_nu -= nu
+        add(specs[isp], nu * -1.0)
+    out = []
+    for key in order:
+        lo, hi = groups[key]
+        out.append((emitted(py2_str(key)),
+                    [emitted(fmt_e16(x)) for x in lo],
+                    [emitted(fmt_e16(x)) for x in hi]))
+    return out, emitted(fmt_e16((chem.PA / chem.RU) ** sum_nu)), sum_nu
+
+
+def build_ir(specs, reacs, name='mech'):
+    """Tables of the mechanism as the reference's emitters would print them."""
+    nsp = len(specs)
+    rev_reacs = [j for j, r in enumerate(reacs) if r.rev]
+    pd_reacs = [j for j, r in enumerate(reacs) if r.thd_body or r.pdep]
+    rxns = []
+    for j, r in enumerate(reacs):
+        if r.cheb or r.plog:
+            raise NotImplementedError('PLOG/Chebyshev reactions are not '
+                                      'supported by the HIP emitter')
+        x = RxnIR(index=j, reac=list(zip(r.reac, r.reac_nu)),
+                  prod=list(zip(r.prod, r.prod_nu)),
+                  arr=arr_ir(r.A, r.b, r.E))
+        if r.rev:
+            x.rev = True
+            x.rev_index = rev_reacs.index(j)
+            if r.rev_par:
+                x.rev_arr = arr_ir(*r.rev_par[0:3])
+            else:
+                x.kc_groups, x.kc_pref, x.kc_dnu = _kc_ir(r, specs)
+        if r.thd_body or r.pdep:
+            x.pd_index = pd_reacs.index(j)
+            x.eff = _eff_terms(r)
+        if r.thd_body:
+            x.thd = True
+        elif r.pdep:
+            if r.pdep_sp != '':
+                x.pdep_sp = int(r.pdep_sp)
+            x.low = arr_ir(*r.low[0:3]) if r.low else None
+            x.high = arr_ir(*r.high[0:3]) if r.high else None
+            x.falloff_low = bool(r.low)
+            x.falloff_high = bool(r.high) and not r.low
+            if r.troe:
+                tp = r.troe_par
+                d = dict(one_minus_a=emitted(fmt_e8(1.0 - tp[0])),
+                         a=emitted(fmt_e8(tp[0])),
+                         t3=emitted(fmt_e8(abs(tp[1]))), t3_pos=tp[1] > 0.0,
+                         t1=emitted(fmt_e8(abs(tp[2]))), t1_pos=tp[2] > 0.0,
+                         t2=None, t2_pos=True)
+                if len(tp) == 4 and tp[3] != 0.0:
+                    d['t2'] = emitted(fmt_e8(abs(tp[3])))
+                    d['t2_pos'] = tp[3] > 0.0
+                x.troe = d
+                x.falloff = 'troe'
+            elif r.sri:
+                sp_ = r.sri_par
+                d = dict(a=emitted(fmt_g6(sp_[0])),
+                         b=emitted(fmt_g6(abs(sp_[1]))), b_pos=sp_[1] > 0.0,
+                         c=emitted(fmt_g6(abs(sp_[2]))), c_pos=sp_[2] > 0.0,
+                         d=None, e=None)
+                if len(sp_) == 5 and sp_[3] != 1.0 and sp_[4] != 0.0:
+                    d['d'] = emitted(fmt_e8(sp_[3]))
+                    d['e'] = emitted(fmt_g6(sp_[4]))
+                x.sri = d
+                x.falloff = 'sri'
+            else:
+                x.falloff = 'lind'
+        for k in sorted(set(r.reac + r.prod)):
+            nu = get_nu(k, r)
+            if nu != 0:
+                x.spec.append((k, nu))
+        rxns.append(x)
+    return MechIR(name=name, nsp=nsp, nr=len(reacs), nrev=len(rev_reacs),
+                  npd=len(pd_reacs),
+                  inv_mw=[emitted(fmt_e16(1.0 / sp.mw)) for sp in specs],
+                  rxns=rxns, species_names=[sp.name for sp in specs])
+
+
+# ---------------------------------------------------------------------------
+# Source generation
+# ---------------------------------------------------------------------------
+
+def _lit(x):
+    """C++ literal that parses to exactly ``x``."""
+    if x == 0.0:
+        return '0.0'
+    r = repr(float(x))
+    if 'e' not in r and '.' not in r and 'n' not in r:
+        r += '.0'
+    return r
+
+
+class _Consts(object):
+    """Constant table: every mechanism number lives in one __constant__ array."""
+
+    def __init__(self, symbol='crs_K'):
+        self.vals = []
+        self.index = {}
+        self.symbol = symbol
+
+    def __call__(self, x):
+        key = float(x).hex()
+        if key not in self.index:
+            self.index[key] = len(self.vals)
+            self.vals.append(float(x))
+        return '{}[{}]'.format(self.symbol, self.index[key])
+
+    def definition(self):
+        body = ',\n  '.join(_lit(v) for v in self.vals) or '0.0'
+        return ('__constant__ double {}[{}] = {{\n  {}\n}};\n'.format(
+            self.symbol, max(len(self.vals), 1), body))
+
+
+def _arr_expr(a, K, logT='logT', T='T'):
+    """FP64 expression of a rate constant, term order as the C text."""
+    if not a.has_exp:
+        s = K(a.pre)
+        for _ in range(a.nT):
+            s = '({} * {})'.format(s, T)
+        return s
+    terms = []
+    if a.logA is not None:
+        terms.append(K(a.logA))
+    if a.b is not None:
+        bt = '{} * {}'.format(K(a.b), logT)
+        terms.append(bt)
+    inner = ' + '.join(terms)
+    if a.E is not None:
+        et = '({} / {})'.format(K(a.E), T)
+        inner = '{} - {}'.format(inner, et) if inner else '-' + et
+    s = 'exp({})'.format(inner)
+    if a.pre is not None:
+        s = '{} * {}'.format(K(a.pre), s)
+    return s
+
+
+def _conc_prod(terms, C='C'):
+    """Concentration product prefix 'C[a] * C[a] * C[b] * ' (left to right)."""
+    s = ''
+    for k, nu in terms:
+        if is_integer(nu):
+            s += '{}[{}] * '.format(C, k) * int(nu)
+        else:
+            s += 'pow({}[{}], {}) * '.format(C, k, _lit(emitted(py2_str(nu))))
+    return s
+
+
+def _kc_expr(x, K, logT='logT', T='T'):
+    """Lines computing Kc for reaction x, as rate_subs.py:593-662."""
+    lines = []
+    first = True
+    for tmid, lo, hi in x.kc_groups:
+        def poly(c):
+            return ('({} + {} * {lt} + {T} * ({} + {T} * ({} + {T} * ({} + {} '
+                    '* {T}))) - {} / {T})'.format(*[K(v) for v in c], lt=logT,
+                                                  T=T))
+        sel = '({T} <= {tm}) ? {lo} : {hi}'.format(T=T, tm=_lit(tmid),
+                                                   lo=poly(lo), hi=poly(hi))
+        lines.append(('Kc = ' if first else 'Kc += ') + '(' + sel + ');')
+        first = False
+    lines.append('Kc = {} * exp(Kc);'.format(K(x.kc_pref)))
+    return lines
+
+
+def _eff_expr(x, K, base='m', C='C'):
+    s = base
+    for k, e in x.eff:
+        if e >= 0:
+            s += ' + {} * {}[{}]'.format(K(e), C, k)
+        else:
+            s += ' - {} * {}[{}]'.format(K(-e), C, k)
+    return s
+
+
+def _pres_mod_lines(x, K, kf_main='kf', C='C'):
+    """Lines ending in 'pm = ...;' for reaction x (rate_subs.py:883-1064)."""
+    out = []
+    if x.thd:
+        out.append('pm = {};'.format(_eff_expr(x, K, C=C)))
+        return out
+    if x.pdep_sp is None:
+        out.append('const double thd = {};'.format(_eff_expr(x, K, C=C)))
+        coll = 'thd'
+    else:
+        coll = '{}[{}]'.format(C, x.pdep_sp)
+    k0 = _arr_expr(x.low, K) if x.low is not None else kf_main
+    kinf = _arr_expr(x.high, K) if x.high is not None else kf_main
+    out.append('const double k0 = {};'.format(k0))
+    out.append('const double kinf = {};'.format(kinf))
+    out.append('const double Pr = k0 * {} / kinf;'.format(coll))
+    tail = ''
+    if x.falloff_low:
+        tail = ' * Pr / (1.0 + Pr)'
+    elif x.falloff_high:
+        tail = ' / (1.0 + Pr)'
+    if x.falloff == 'troe':
+        t = x.troe
+        e1 = ('exp(-T / {})' if t['t3_pos'] else 'exp(T / {})').format(K(t['t3']))
+        e2 = ('exp(-T / {})' if t['t1_pos'] else 'exp(T / {})').format(K(t['t1']))
+        fc = '{} * {} + {} * {}'.format(K(t['one_minus_a']), e1, K(t['a']), e2)
+        if t['t2'] is not None:
+            fc += ' + ' + ('exp(-{} / T)' if t['t2_pos'] else
+                           'exp({} / T)').format(K(t['t2']))
+        out.append('const double logFcent = log10(fmax({}, 1.0e-300));'.format(fc))
+        out.append('const double lPr = log10(fmax(Pr, 1.0e-300));')
+        out.append('const double A = lPr - 0.67 * logFcent - 0.4;')
+        out.append('const double B = 0.806 - 1.1762 * logFcent - 0.14 * lPr;')
+        out.append('pm = exp10(logFcent / (1.0 + A * A / (B * B))){};'.format(tail))
+    elif x.falloff == 'sri':
+        s = x.sri
+        out.append('const double lPr = log10(fmax(Pr, 1.0e-300));')
+        out.append('const double X = 1.0 / (1.0 + lPr * lPr);')
+        e1 = ('exp(-{} / T)' if s['b_pos'] else 'exp({} / T)').format(K(s['b']))
+        e2 = ('exp(-T / {})' if s['c_pos'] else 'exp(T / {})').format(K(s['c']))
+        f = 'pow({} * {} + {}, X)'.format(K(s['a']), e1, e2)
+        if s['d'] is not None:
+            f += ' * {} * pow(T, {})'.format(K(s['d']), K(s['e']))
+        out.append('pm = {}{};'.format(f, tail))
+    else:
+        if x.falloff_low:
+            out.append('pm = Pr / (1.0 + Pr);')
+        elif x.falloff_high:
+            out.append('pm = 1.0 / (1.0 + Pr);')
+        else:
+            out.append('pm = 1.0;')
+    return out
+
+
+def _rate_lines(x, K, C='C'):
+    """Lines for kf, f and (if reversible) Kc and r of reaction x."""
+    out = ['const double kf = {};'.format(_arr_expr(x.arr, K))]
+    out.append('const double f = {}kf;'.format(_conc_prod(x.reac, C)))
+    if x.rev:
+        if x.rev_arr is not None:
+            out.append('const double r = {}{};'.format(
+                _conc_prod(x.prod, C), _arr_expr(x.rev_arr, K)))
+        else:
+            out.append('double Kc;')
+            out.extend(_kc_expr(x, K))
+            out.append('const double r = {}kf / Kc;'.format(
+                _conc_prod(x.prod, C)))
+    return out
+
+
+def _spec_lines(x, seen, W='w', f='f', r='r', pm='pm'):
+    """Species-rate updates of reaction x (rate_subs.py:1162-1217)."""
+    out = []
+    for k, nu in x.spec:
+        anu = abs(nu)
+        coef = ''
+        if anu != 1.0:
+            # integral: '{}'.format(float(nu)); else '{:3}' (rate_subs.py:1189-1193)
+            coef = '{} * '.format(_lit(emitted(py2_str(float(anu)))))
+        term = '({} - {})'.format(f, r) if x.rev else f
+        if x.pd_index is not None:
+            term += ' * ' + pm
+        if seen[k]:
+            op = '-' if nu < 0 else '+'
+            out.append('{W}[{k}] = {W}[{k}] {op} {c}{t};'.format(
+                W=W, k=k, op=op, c=coef, t=term))
+        else:
+            neg = '-' if nu < 0 else ''
+            if coef:
+                out.append('{W}[{k}] = {n}{c}{t};'.format(
+                    W=W, k=k, n=neg, c=coef, t=term))
+            else:
+                out.append('{W}[{k}] = {n}{t};'.format(W=W, k=k, n=neg, t=term))
+            seen[k] = True
+    return out
+
+
+def _conc_lines(ir, K, y='y', C='C'):
+    """eval_conc body (rate_subs.py:1292-1391), same operation order."""
+    nsp = ir.nsp
+    out = []
+    s = ' + '.join('{}[{}]'.format(y, k) for k in range(nsp - 1))
+    out.append('const double y_N = 1.0 - ({});'.format(s))
+    s = ' + '.join('({}[{}] * {})'.format(y, k, K(ir.inv_mw[k]))
+                   for k in range(nsp - 1))
+    s += ' + (y_N * {})'.format(K(ir.inv_mw[nsp - 1]))
+    out.append('double mw_avg = {};'.format(s))
+    out.append('mw_avg = 1.0 / mw_avg;')
+    out.append('const double rho = P * mw_avg / ({} * T);'.format(
+        _lit(emitted(fmt_e8(chem.RU)))))
+    out.append('double {}[{}];'.format(C, nsp))
+    for k in range(nsp - 1):
+        out.append('{C}[{k}] = rho * {y}[{k}] * {w};'.format(
+            C=C, k=k, y=y, w=K(ir.inv_mw[k])))
+    out.append('{C}[{k}] = rho * y_N * {w};'.format(
+        C=C, k=nsp - 1, w=K(ir.inv_mw[nsp - 1])))
+    return out
+
+
+def _indent(lines, n):
+    pad = ' ' * n
+    return ''.join(pad + l + '\n' for l in lines)
+
+
+def emit_hip_source(ir):
+    """Full HIP source text for mechanism ``ir``."""
+    K = _Consts()
+    nsp, nr, nrev, npd = ir.nsp, ir.nr, ir.nrev, ir.npd
+    ru = _lit(emitted(fmt_e8(chem.RU)))
+    body = []
+
+    # ---- stage kernel: eval_conc -------------------------------------
+    lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
+             'if (i >= n) return;',
+             'const double T = T_[i];', 'const double P = P_[i];',
+             'double y[{}];'.format(max(nsp - 1, 1))]
+    lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k) for k in range(nsp - 1)]
+    lines += _conc_lines(ir, K)
+    lines += ['y_N_out[i] = y_N;', 'mw_out[i] = mw_avg;', 'rho_out[i] = rho;']
+    lines += ['Cout[{k} * ld + i] = C[{k}];'.format(k=k) for k in range(nsp)]
+    body.append(
+        '__global__ __launch_bounds__(crs::kBlock) void k_eval_conc(\n'
+        '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Y, double *__restrict__ y_N_out,\n'
+        '    double *__restrict__ mw_out, double *__restrict__ rho_out,\n'
+        '    double *__restrict__ Cout, int64_t n, int64_t ld)\n{\n'
+        '#pragma clang fp contract(off)\n' + _indent(lines, 4) + '}\n\n')
+
+    # ---- stage kernel: eval_rxn_rates ---------------------------------
+    lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
+             'if (i >= n) return;',
+             'const double T = T_[i];',
+             'const double logT = log(T);',
+             'double C[{}];'.format(nsp)]
+    lines += ['C[{k}] = Cin[{k} * ld + i];'.format(k=k) for k in range(nsp)]
+    for x in ir.rxns:
+        lines.append('{  // rxn %d' % x.index)
+        inner = _rate_lines(x, K)
+        inner.append('fwd[{} * ld + i] = f;'.format(x.index))
+        if x.rev:
+            inner.append('rev[{} * ld + i] = r;'.format(x.rev_index))
+        lines += ['    ' + l for l in inner]
+        lines.append('}')
+    body.append(
+        '__global__ __launch_bounds__(crs::kBlock) void k_eval_rxn_rates(\n'
+        '    const double *__restrict__ T_, const double *__restrict__ Cin,\n'
+        '    double *__restrict__ fwd, double *__restrict__ rev, int64_t n,\n'
+        '    int64_t ld)\n{\n' + _indent(lines, 4) + '}\n\n')
+
+    # ---- stage kernel: get_rxn_pres_mod -------------------------------
+    if npd:
+        lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
+                 'if (i >= n) return;',
+                 'const double T = T_[i];', 'const double P = P_[i];',
+                 'const double logT = log(T);',
+                 'const double m = P / ({} * T);'.format(ru),
+                 'double C[{}];'.format(nsp)]
+        lines += ['C[{k}] = Cin[{k} * ld + i];'.format(k=k) for k in range(nsp)]
+        for x in ir.rxns:
+            if x.pd_index is None:
+                continue
+            lines.append('{  // rxn %d' % x.index)
+            inner = ['double pm;']
+            need_kf = (not x.thd) and (x.low is None or x.high is None)
+            if need_kf:
+                inner.append('const double kf = {};'.format(_arr_expr(x.arr, K)))
+            inner += _pres_mod_lines(x, K)
+            inner.append('pres_mod[{} * ld + i] = pm;'.format(x.pd_index))
+            lines += ['    ' + l for l in inner]
+            lines.append('}')
+        lines.append('(void)logT; (void)m;')
+        body.append(
+            '__global__ __launch_bounds__(crs::kBlock) void k_get_rxn_pres_mod(\n'
+            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Cin, double *__restrict__ pres_mod,\n'
+            '    int64_t n, int64_t ld)\n{\n' + _indent(lines, 4) + '}\n\n')
+
+    # ---- stage kernel: eval_spec_rates --------------------------------
+    lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
+             'if (i >= n) return;',
+             'double w[{}];'.format(nsp)]
+    seen = [False] * nsp
+    for x in ir.rxns:
+        inner = ['const double f = fwd[{} * ld + i];'.format(x.index)]
+        if x.rev:
+            inner.append('const double r = rev[{} * ld + i];'.format(x.rev_index))
+        if x.pd_index is not None:
+            inner.append('const double pm = pres_mod[{} * ld + i];'.format(x.pd_index))
+        inner += _spec_lines(x, seen)
+        lines.append('{  // rxn %d' % x.index)
+        lines += ['    ' + l for l in inner]
+        lines.append('}')
+    for k in range(nsp):
+        if not seen[k]:
+            lines.append('w[{}] = 0.0;'.format(k))
+    lines += ['sp_rates[{k} * ld + i] = w[{k}];'.format(k=k)
+              for k in range(nsp - 1)]
+    lines.append('dy_N[i] = w[{}];'.format(nsp - 1))
+    body.append(
+        '__global__ __launch_bounds__(crs::kBlock) void k_eval_spec_rates(\n'
+        '    const double *__restrict__ fwd, const double *__restrict__ rev,\n'
+        '    const double *__restrict__ pres_mod, double *__restrict__ sp_rates,\n'
+        '    double *__restrict__ dy_N, int64_t n, int64_t ld)\n{\n'
+        '#pragma clang fp contract(off)\n' + _indent(lines, 4) + '}\n\n')
+
+    # ---- fused FP64 per-state body ------------------------------------
+    lines = ['const double T = T_[i];', 'const double P = P_[i];',
+             'double y[{}];'.format(max(nsp - 1, 1))]
+    lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k) for k in range(nsp - 1)]
+    lines += _conc_lines(ir, K)
+    lines += ['const double logT = log(T);',
+              'const double m = P / ({} * T);'.format(ru),
+              '(void)m;',
+              'double w[{}];'.format(nsp)]
+    seen = [False] * nsp
+    for x in ir.rxns:
+        inner = _rate_lines(x, K)
+        if x.pd_index is not None:
+            inner.append('double pm;')
+            inner += _pres_mod_lines(x, K)
+        inner += _spec_lines(x, seen)
+        lines.append('{  // rxn %d' % x.index)
+        lines += ['    ' + l for l in inner]
+        lines.append('}')
+    for k in range(nsp):
+        if not seen[k]:
+            lines.append('w[{}] = 0.0;'.format(k))
+    lines += ['W[{k} * ld + i] = w[{k}];'.format(k=k) for k in range(nsp)]
+    body.append(
+        '__device__ __forceinline__ void wdot_f64_state(\n'
+        '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Y, double *__restrict__ W,\n'
+        '    int64_t ld, int64_t i)\n{\n' + _indent(lines, 4) + '}\n\n')
+    body.append(
+        '__global__ __launch_bounds__(crs::kBlock) void k_wdot_f64(\n'
+        '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
+        '    int64_t ld)\n{\n'
+        '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
+        '    if (i >= n) return;\n'
+        '    wdot_f64_state(T_, P_, Y, W, ld, i);\n}\n\n')
+    body.append(
+        '__global__ __launch_bounds__(crs::kBlock) void k_wdot_f64_subset(\n'
+        '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Y, double *__restrict__ W,\n'
+        '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
+        '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
+        '    if (t >= m) return;\n'
+        '    const int64_t i = idx[t];\n'
+        '    if (i < 0 || i >= ld) return;\n'
+        '    wdot_f64_state(T_, P_, Y, W, ld, i);\n}\n\n')
+
+    header = (
+        '// Generated by create_rate_subs_amd.hip_emitter for mechanism '
+        '"{name}".\n'
+        '// {nsp} species, {nr} reactions ({nrev} reversible, {npd} '
+        'third-body/falloff).\n'
+        '#include "crs_common.h"\n\n'
+        'namespace {{\n\n'
+        'constexpr int NSP = {nsp};\nconstexpr int NR = {nr};\n'
+        'constexpr int NREV = {nrev};\nconstexpr int NPD = {npd};\n\n'
+        .format(name=ir.name, nsp=nsp, nr=nr, nrev=nrev, npd=npd))
+    consts = K.definition() + '\n'
+    abi = _abi_text(ir)
+    return header + consts + ''.join(body) + '}  // namespace\n\n' + abi
+
+
+def _abi_text(ir):
+    npd = ir.npd
+    pm_launch = ('    k_get_rxn_pres_mod<<<crs::grid_for(n), crs::kBlock, 0, s>>>'
+                 '(T, P, C, pres_mod, n, ld);\n' if npd else '')
+    return '''extern "C" {{
+
+int crs_get_mech_info(crs_mech_info *out)
+{{
+    if (out == nullptr)
+        return crs::fail(CRS_ERR_ARG, "crs_get_mech_info: out is NULL");
+    out->nsp = NSP;
+    out->nr = NR;
+    out->nrev = NREV;
+    out->npd = NPD;
+    out->modes = (1 << CRS_MODE_FP64);
+    out->name = "{name}";
+    return CRS_OK;
+}}
+
+const char *crs_last_error(void) {{ return crs::error_buffer(); }}
+
+int crs_eval_conc(const double *T, const double *P, const double *Y,
+                  double *y_N, double *mw_avg, double *rho, double *conc,
+                  int64_t n, int64_t ld, void *stream)
+{{
+    const char *fn = "crs_eval_conc";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    CRS_TRY(crs::check_ptr(fn, T, "T", n));
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, Y, "Y", n));
+    CRS_TRY(crs::check_ptr(fn, y_N, "y_N", n));
+    CRS_TRY(crs::check_ptr(fn, mw_avg, "mw_avg", n));
+    CRS_TRY(crs::check_ptr(fn, rho, "rho", n));
+    CRS_TRY(crs::check_ptr(fn, conc, "conc", n));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    k_eval_conc<<<crs::grid_for(n), crs::kBlock, 0, s>>>(T, P, Y, y_N, mw_avg,
+                                                         rho, conc, n, ld);
+    return crs::launched(fn);
+}}
+
+int crs_eval_rxn_rates(const double *T, const double *P, const double *C,
+                       double *fwd, double *rev, int64_t n, int64_t ld,
+                       void *stream)
+{{
+    const char *fn = "crs_eval_rxn_rates";
+    (void)P;
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    CRS_TRY(crs::check_ptr(fn, T, "T", n));
+    CRS_TRY(crs::check_ptr(fn, C, "C", n));
+    CRS_TRY(crs::check_ptr(fn, fwd, "fwd", n));
+    CRS_TRY(crs::check_ptr(fn, rev, "rev", NREV > 0 ? n : 0));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    k_eval_rxn_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(T, C, fwd, rev,
+                                                              n, ld);
+    return crs::launched(fn);
+}}
+
+int crs_get_rxn_pres_mod(const double *T, const double *P, const double *C,
+                         double *pres_mod, int64_t n, int64_t ld,
+                         void *stream)
+{{
+    const char *fn = "crs_get_rxn_pres_mod";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (NPD == 0) return CRS_OK;
+    CRS_TRY(crs::check_ptr(fn, T, "T", n));
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, C, "C", n));
+    CRS_TRY(crs::check_ptr(fn, pres_mod, "pres_mod", n));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+{pm_launch}    return crs::launched(fn);
+}}
+
+int crs_eval_spec_rates(const double *fwd, const double *rev,
+                        const double *pres_mod, double *sp_rates,
+                        double *dy_N, int64_t n, int64_t ld, void *stream)
+{{
+    const char *fn = "crs_eval_spec_rates";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    CRS_TRY(crs::check_ptr(fn, fwd, "fwd", n));
+    CRS_TRY(crs::check_ptr(fn, rev, "rev", NREV > 0 ? n : 0));
+    CRS_TRY(crs::check_ptr(fn, pres_mod, "pres_mod", NPD > 0 ? n : 0));
+    CRS_TRY(crs::check_ptr(fn, sp_rates, "sp_rates", NSP > 1 ? n : 0));
+    CRS_TRY(crs::check_ptr(fn, dy_N, "dy_N", n));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    k_eval_spec_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(
+        fwd, rev, pres_mod, sp_rates, dy_N, n, ld);
+    return crs::launched(fn);
+}}
+
+static int check_mode(const char *fn, int mode)
+{{
+    if (mode == CRS_MODE_FP64) return CRS_OK;
+    if (mode == CRS_MODE_MIXED)
+        return crs::fail(CRS_ERR_UNSUPPORTED,
+                         "%s: mode %d not compiled into this library", fn, mode);
+    return crs::fail(CRS_ERR_ARG, "%s: unknown mode %d", fn, mode);
+}}
+
+int crs_eval_wdot(const double *T, const double *P, const double *Y,
+                  double *wdot, int64_t n, int64_t ld, int mode, void *stream)
+{{
+    const char *fn = "crs_eval_wdot";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    CRS_TRY(check_mode(fn, mode));
+    CRS_TRY(crs::check_ptr(fn, T, "T", n));
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, Y, "Y", n));
+    CRS_TRY(crs::check_ptr(fn, wdot, "wdot", n));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    k_wdot_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(T, P, Y, wdot, n, ld);
+    return crs::launched(fn);
+}}
+
+int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
+                         double *wdot, const int32_t *idx, int64_t m,
+                         int64_t ld, int mode, void *stream)
+{{
+    const char *fn = "crs_eval_wdot_subset";
+    if (m < 0)
+        return crs::fail(CRS_ERR_ARG, "%s: m must be >= 0", fn);
+    if (ld < 0)
+        return crs::fail(CRS_ERR_ARG, "%s: ld must be >= 0", fn);
+    CRS_TRY(check_mode(fn, mode));
+    CRS_TRY(crs::check_ptr(fn, T, "T", m));
+    CRS_TRY(crs::check_ptr(fn, P, "P", m));
+    CRS_TRY(crs::check_ptr(fn, Y, "Y", m));
+    CRS_TRY(crs::check_ptr(fn, wdot, "wdot", m));
+    CRS_TRY(crs::check_ptr(fn, idx, "idx", m));
+    if (m == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    k_wdot_f64_subset<<<crs::grid_for(m), crs::kBlock, 0, s>>>(T, P, Y, wdot,
+                                                               idx, m, ld);
+    return crs::launched(fn);
+}}
+
+}}  // extern "C"
+'''.format(name=ir.name, pm_launch=pm_launch)
+
+
+def write_hip(out_dir, specs, reacs, name='mech', spec_order=None):
+    """Emit ``crs_<name>.hip`` into ``out_dir``; return its path."""
+    ir = build_ir(specs, reacs, name=name)
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, 'crs_{}.hip'.format(name))
+    text = emit_hip_source(ir)
+    old = None
+    if os.path.exists(path):
+        with open(path) as fh:
+            old = fh.read()
+    if old != text:
+        with open(path, 'w') as fh:
+            fh.write(text)
+    return path

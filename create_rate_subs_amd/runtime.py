@@ -1,0 +1,199 @@
+"""ctypes binding of a mechanism library, on torch device tensors.
+
+``load(path)`` opens one ``libcrs_<mech>.so`` (see include/crs_hip.h) and
+returns a :class:`MechLib` whose methods take and return torch FP64 tensors
+on the current GPU and enqueue on torch's current stream.  There is no
+fallback: a missing library, a missing GPU or a failing call raises.
+
+torch is imported before the library is opened so that the library binds
+to the HIP runtime torch already loaded (same ``libamdhip64.so.7`` soname)
+and shares its device memory and streams.
+"""
+
+import ctypes
+import os
+
+import torch
+
+__all__ = ['CrsError', 'MechLib', 'load', 'MODES', 'EXPORTED_SYMBOLS']
+
+MODES = {'fp64': 0, 'mixed': 1}
+
+# Every entry point include/crs_hip.h declares.
+EXPORTED_SYMBOLS = ('crs_get_mech_info', 'crs_last_error', 'crs_eval_conc',
+                    'crs_eval_rxn_rates', 'crs_get_rxn_pres_mod',
+                    'crs_eval_spec_rates', 'crs_eval_wdot',
+                    'crs_eval_wdot_subset')
+
+
+class CrsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__('[crs error {}] {}'.format(code, msg))
+        self.code = code
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [('nsp', ctypes.c_int32), ('nr', ctypes.c_int32),
+                ('nrev', ctypes.c_int32), ('npd', ctypes.c_int32),
+                ('modes', ctypes.c_int32), ('name', ctypes.c_char_p)]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_SIGS = {
+    'crs_get_mech_info': [ctypes.POINTER(_Info)],
+    'crs_eval_conc': [_P] * 7 + [_I64, _I64, _P],
+    'crs_eval_rxn_rates': [_P] * 5 + [_I64, _I64, _P],
+    'crs_get_rxn_pres_mod': [_P] * 4 + [_I64, _I64, _P],
+    'crs_eval_spec_rates': [_P] * 5 + [_I64, _I64, _P],
+    'crs_eval_wdot': [_P] * 4 + [_I64, _I64, ctypes.c_int, _P],
+    'crs_eval_wdot_subset': [_P] * 5 + [_I64, _I64, ctypes.c_int, _P],
+}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class MechLib(object):
+    """One mechanism library.  Sizes: ``nsp``, ``nr``, ``nrev``, ``npd``."""
+
+    def __init__(self, path):
+        if not os.path.exists(path):
+            raise FileNotFoundError('mechanism library not built: ' + path)
+        self.path = os.path.abspath(path)
+        self.lib = ctypes.CDLL(self.path)
+        for name in EXPORTED_SYMBOLS:
+            fn = getattr(self.lib, name)
+            fn.restype = ctypes.c_int
+            if name in _SIGS:
+                fn.argtypes = _SIGS[name]
+        self.lib.crs_last_error.restype = ctypes.c_char_p
+        self.lib.crs_last_error.argtypes = []
+        info = _Info()
+        self._check(self.lib.crs_get_mech_info(ctypes.byref(info)))
+        self.nsp, self.nr = info.nsp, info.nr
+        self.nrev, self.npd = info.nrev, info.npd
+        self.modes = info.modes
+        self.name = info.name.decode()
+
+    # -- helpers -----------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise CrsError(rc, self.lib.crs_last_error().decode())
+
+    @staticmethod
+    def _vec(x, n, what):
+        if not (x.is_cuda and x.dtype == torch.float64 and x.dim() == 1 and
+                x.is_contiguous() and x.numel() == n):
+            raise ValueError('{} must be a contiguous cuda float64 vector '
+                             'of length {}'.format(what, n))
+
+    @staticmethod
+    def _rows(x, rows, n, what):
+        if not (x.is_cuda and x.dtype == torch.float64 and x.dim() == 2 and
+                x.shape[0] == rows and x.shape[1] == n and x.stride(1) == 1):
+            raise ValueError('{} must be a cuda float64 [{}][{}] array with '
+                             'unit column stride'.format(what, rows, n))
+        return x.stride(0) if rows > 1 else max(x.stride(0), n)
+
+    def _new(self, rows, n, ld=None):
+        ld = ld or n
+        buf = torch.empty((max(rows, 1), ld), dtype=torch.float64,
+                          device='cuda')
+        return buf[:rows, :n]
+
+    # -- the four batched reference functions -----------------------------
+    def eval_conc(self, T, P, Y):
+        n = T.numel()
+        self._vec(T, n, 'T')
+        self._vec(P, n, 'P')
+        ld = self._rows(Y, self.nsp, n, 'Y')
+        y_N = torch.empty(n, dtype=torch.float64, device='cuda')
+        mw = torch.empty_like(y_N)
+        rho = torch.empty_like(y_N)
+        C = self._new(self.nsp, n, ld)
+        self._check(self.lib.crs_eval_conc(_ptr(T), _ptr(P), _ptr(Y),
+                                           _ptr(y_N), _ptr(mw), _ptr(rho),
+                                           _ptr(C), n, ld, _stream()))
+        return dict(y_N=y_N, mw=mw, rho=rho, C=C)
+
+    def eval_rxn_rates(self, T, P, C):
+        n = T.numel()
+        self._vec(T, n, 'T')
+        self._vec(P, n, 'P')
+        ld = self._rows(C, self.nsp, n, 'C')
+        fwd = self._new(self.nr, n, ld)
+        rev = self._new(self.nrev, n, ld)
+        self._check(self.lib.crs_eval_rxn_rates(
+            _ptr(T), _ptr(P), _ptr(C), _ptr(fwd), _ptr(rev), n, ld,
+            _stream()))
+        return fwd, rev
+
+    def get_rxn_pres_mod(self, T, P, C):
+        n = T.numel()
+        self._vec(T, n, 'T')
+        self._vec(P, n, 'P')
+        ld = self._rows(C, self.nsp, n, 'C')
+        pm = self._new(self.npd, n, ld)
+        self._check(self.lib.crs_get_rxn_pres_mod(
+            _ptr(T), _ptr(P), _ptr(C), _ptr(pm), n, ld, _stream()))
+        return pm
+
+    def eval_spec_rates(self, fwd, rev, pm):
+        n = fwd.shape[1]
+        ld = self._rows(fwd, self.nr, n, 'fwd')
+        if self.nrev:
+            if self._rows(rev, self.nrev, n, 'rev') != ld:
+                raise ValueError('rev and fwd must share the row stride')
+        if self.npd:
+            if self._rows(pm, self.npd, n, 'pres_mod') != ld:
+                raise ValueError('pres_mod and fwd must share the row stride')
+        W = self._new(self.nsp, n, ld)
+        self._check(self.lib.crs_eval_spec_rates(
+            _ptr(fwd), _ptr(rev) if self.nrev else None,
+            _ptr(pm) if self.npd else None, _ptr(W),
+            ctypes.c_void_p(W.data_ptr() + 8 * (self.nsp - 1) * ld), n, ld,
+            _stream()))
+        return W
+
+    # -- fused path --------------------------------------------------------
+    def eval_wdot(self, T, P, Y, mode='fp64', out=None):
+        n = T.numel()
+        self._vec(T, n, 'T')
+        self._vec(P, n, 'P')
+        ld = self._rows(Y, self.nsp, n, 'Y')
+        W = out if out is not None else self._new(self.nsp, n, ld)
+        if self._rows(W, self.nsp, n, 'wdot') != ld:
+            raise ValueError('wdot and Y must share the row stride')
+        self._check(self.lib.crs_eval_wdot(_ptr(T), _ptr(P), _ptr(Y), _ptr(W),
+                                           n, ld, MODES[mode], _stream()))
+        return W
+
+    def eval_wdot_subset(self, T, P, Y, W, idx, mode='fp64'):
+        n = T.numel()
+        ld = self._rows(Y, self.nsp, n, 'Y')
+        if self._rows(W, self.nsp, n, 'wdot') != ld:
+            raise ValueError('wdot and Y must share the row stride')
+        if not (idx.is_cuda and idx.dtype == torch.int32 and idx.dim() == 1
+                and idx.is_contiguous()):
+            raise ValueError('idx must be a contiguous cuda int32 vector')
+        self._check(self.lib.crs_eval_wdot_subset(
+            _ptr(T), _ptr(P), _ptr(Y), _ptr(W), _ptr(idx), idx.numel(), ld,
+            MODES[mode], _stream()))
+        return W
+
+
+_LOADED = {}
+
+
+def load(path):
+    """Open (once per process) the mechanism library at ``path``."""
+    path = os.path.abspath(path)
+    if path not in _LOADED:
+        _LOADED[path] = MechLib(path)
+    return _LOADED[path]

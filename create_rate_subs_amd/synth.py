@@ -1,0 +1,307 @@
+"""Seeded synthetic mechanisms, written as real Chemkin and NASA-7 text.
+
+The reference ships no mechanism, so every workload is generated.  The
+generator writes mechanism/thermo *files* so that the parser (and with it
+every unit conversion) is exercised end to end, exactly as a user's
+mechanism would be.
+
+Species are C/H/O compositions drawn from a small lattice (so that many
+element-balanced reactions exist) plus N2 as the bath gas in the last
+position; N2 takes part in reactions only as a collider.  Every reaction is
+element-balanced, which the mass- and element-conservation invariants and
+the equilibrium construction of near-equilibrium states rely on.
+
+Thermodynamics: cp/R(T) = 2.5 + c (1 - exp(-T/theta)) with c ~ U(0.5, 8)
+and theta ~ U(500, 3000) K is least-squares fitted by a quartic on
+[300, 1000] and [1000, 5000] K; a5/a6 of the low range reproduce
+h298 ~ N(0, 150 kJ/mol) and s298 ~ U(150, 300) J/mol/K, those of the high
+range make H and S continuous at T_mid = 1000 K.
+
+Rates: log10 A ~ U(8, 16) (cgs/mole), b ~ U(-2, 3), Ea ~ U(0, 60) kcal/mol
+(written in cal/mole, the Chemkin default).  Printed parameters are
+quantised (b to 3 decimals, efficiencies to 2, Troe/SRI to 6 significant
+digits) so that Python 2 and Python 3 spellings of every emitted constant
+coincide.
+"""
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = ['MechConfig', 'MECHS', 'generate_mechanism', 'write_mechanism']
+
+_ELEMS = ('C', 'H', 'O')
+
+
+@dataclass(frozen=True)
+class MechConfig:
+    """Recipe for one synthetic mechanism."""
+    name: str
+    n_species: int
+    n_reactions: int
+    frac_rev: float
+    frac_thd: float
+    n_enhanced: int
+    frac_troe: float = 0.0
+    frac_lind: float = 0.0
+    frac_sri: float = 0.0
+    seed: int = 0
+
+
+# The three mechanisms the workloads are built on.
+MECHS = {
+    'mech13': MechConfig('mech13', 13, 27, 0.85, 0.15, 3, seed=1000),
+    'mech53': MechConfig('mech53', 53, 325, 0.80, 0.10, 5, frac_troe=0.08,
+                         frac_lind=0.02, seed=1001),
+    'mech111': MechConfig('mech111', 111, 784, 0.80, 0.12, 5, frac_troe=0.10,
+                          frac_sri=0.03, seed=1003),
+}
+
+
+def _species_name(comp):
+    name = ''
+    for el, n in zip(_ELEMS, comp):
+        if n:
+            name += el + (str(n) if n > 1 else '')
+    return name
+
+
+def _pick_compositions(rng, count):
+    """``count`` distinct (nC, nH, nO) compositions, smallest molecules first."""
+    lattice = [(c, h, o) for c in range(5) for h in range(9)
+               for o in range(5) if c + h + o > 0]
+    # order by atom count, random within a shell
+    keys = rng.random(len(lattice))
+    lattice = [x for _, x in sorted(zip(keys, lattice))]
+    lattice.sort(key=lambda x: sum(x))
+    return lattice[:count]
+
+
+def _fit_nasa(rng):
+    """Random smooth cp/R -> NASA-7 (lo, hi) coefficient lists."""
+    c = rng.uniform(0.5, 8.0)
+    theta = rng.uniform(500.0, 3000.0)
+    h298 = rng.normal(0.0, 150.0e3)           # J/mol
+    s298 = rng.uniform(150.0, 300.0)          # J/mol/K
+    R = 8.3144621
+
+    def cp(T):
+        return 2.5 + c * (1.0 - np.exp(-T / theta))
+
+    def fit(t0, t1):
+        T = np.linspace(t0, t1, 200)
+        Ts = T / 1000.0
+        V = np.vander(Ts, 5, increasing=True)
+        a, *_ = np.linalg.lstsq(V, cp(T), rcond=None)
+        return [a[i] / 1000.0 ** i for i in range(5)]
+
+    lo = fit(300.0, 1000.0)
+    hi = fit(1000.0, 5000.0)
+
+    def h_poly(a, T):      # H/R without a5
+        return (a[0] * T + a[1] * T ** 2 / 2 + a[2] * T ** 3 / 3 +
+                a[3] * T ** 4 / 4 + a[4] * T ** 5 / 5)
+
+    def s_poly(a, T):      # S/R without a6
+        return (a[0] * math.log(T) + a[1] * T + a[2] * T ** 2 / 2 +
+                a[3] * T ** 3 / 3 + a[4] * T ** 4 / 4)
+
+    T0, Tm = 298.15, 1000.0
+    lo5 = h298 / R - h_poly(lo, T0)
+    lo6 = s298 / R - s_poly(lo, T0)
+    hi5 = h_poly(lo, Tm) + lo5 - h_poly(hi, Tm)
+    hi6 = s_poly(lo, Tm) + lo6 - s_poly(hi, Tm)
+    return lo + [lo5, lo6], hi + [hi5, hi6]
+
+
+def _fmt_coef(x):
+    return '{:15.8E}'.format(x)
+
+
+def _thermo_entry(name, elem_counts, lo, hi):
+    """Four fixed-column NASA-7 lines (columns as read by read_thermo)."""
+    elem = ''
+    for el, n in elem_counts:
+        elem += '{:<2}{:>3}'.format(el, n)
+    elem = '{:<20}'.format(elem)
+    l1 = '{:<18}{:<6}{}G{:>10.3f}{:>10.3f}{:>8.2f}'.format(
+        name, 'SYN', elem, 300.0, 5000.0, 1000.0)
+    l1 = '{:<79}1'.format(l1)
+    l2 = ''.join(_fmt_coef(x) for x in hi[0:5]) + '    2'
+    l3 = ''.join(_fmt_coef(x) for x in (hi[5], hi[6], lo[0], lo[1], lo[2]))
+    l3 += '    3'
+    l4 = ''.join(_fmt_coef(x) for x in lo[3:7]) + '{:>15}'.format('') + '    4'
+    return '\n'.join([l1, l2, l3, l4]) + '\n'
+
+
+@dataclass
+class _Rxn:
+    reac: list
+    prod: list
+    rev: bool
+    kind: str = 'elem'          # elem | thd | troe | lind | sri
+    arr: tuple = ()
+    low: tuple = ()
+    troe: tuple = ()
+    sri: tuple = ()
+    eff: list = field(default_factory=list)
+
+
+def _side_text(terms, names):
+    out = []
+    for sp, nu in terms:
+        out.append(('2' if nu == 2 else '') + names[sp])
+    return ' + '.join(out)
+
+
+def _arrhenius(rng):
+    logA = rng.uniform(8.0, 16.0)
+    A = float('%.6E' % (10.0 ** logA))
+    b = round(rng.uniform(-2.0, 3.0), 3)
+    Ea = round(rng.uniform(0.0, 60.0) * 1000.0, 3)    # cal/mol
+    return A, b, Ea
+
+
+def generate_mechanism(cfg):
+    """Build the mechanism described by ``cfg``; return (mech, thermo) text."""
+    rng = np.random.default_rng(cfg.seed)
+    comps = _pick_compositions(rng, cfg.n_species - 1)
+    names = [_species_name(c) for c in comps] + ['N2']
+    nsp = len(names)
+    vec = [np.array(c) for c in comps]
+
+    # index every 1- and 2-term product side by its element vector
+    sides = {}
+    ns = nsp - 1
+    for i in range(ns):
+        for nu in (1, 2):
+            sides.setdefault(tuple(vec[i] * nu), []).append(((i, nu),))
+        for j in range(i + 1, ns):
+            for nu_i in (1, 2):
+                for nu_j in (1, 2):
+                    key = tuple(vec[i] * nu_i + vec[j] * nu_j)
+                    sides.setdefault(key, []).append(((i, nu_i), (j, nu_j)))
+
+    def coef():
+        return 2 if rng.random() < 0.1 else 1
+
+    seen = set()
+    balanced = []
+    tries = 0
+    while len(balanced) < cfg.n_reactions:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError('could not draw enough balanced reactions')
+        n_r = 1 + int(rng.random() < 0.6)
+        picks = rng.choice(ns, size=n_r, replace=False)
+        reac = tuple(sorted((int(i), coef()) for i in picks))
+        key = tuple(sum(vec[i] * nu for i, nu in reac))
+        cands = sides.get(key, [])
+        rs = {i for i, _ in reac}
+        cands = [p for p in cands if not ({i for i, _ in p} & rs)]
+        if not cands:
+            continue
+        prod = cands[int(rng.integers(len(cands)))]
+        sig = (reac, tuple(prod))
+        if sig in seen or (tuple(prod), reac) in seen:
+            continue
+        seen.add(sig)
+        balanced.append((list(reac), list(prod)))
+
+    nr = cfg.n_reactions
+    n_rev = round(cfg.frac_rev * nr)
+    n_thd = round(cfg.frac_thd * nr)
+    n_troe = round(cfg.frac_troe * nr)
+    n_lind = round(cfg.frac_lind * nr)
+    n_sri = round(cfg.frac_sri * nr)
+    kinds = (['thd'] * n_thd + ['troe'] * n_troe + ['lind'] * n_lind +
+             ['sri'] * n_sri)
+    kinds += ['elem'] * (nr - len(kinds))
+    kinds = [kinds[i] for i in rng.permutation(nr)]
+    rev_flags = np.zeros(nr, dtype=bool)
+    rev_flags[rng.permutation(nr)[:n_rev]] = True
+
+    rxns = []
+    for j, (reac, prod) in enumerate(balanced):
+        r = _Rxn(reac=reac, prod=prod, rev=bool(rev_flags[j]), kind=kinds[j])
+        r.arr = _arrhenius(rng)
+        if r.kind != 'elem':
+            picks = rng.choice(nsp, size=cfg.n_enhanced, replace=False)
+            r.eff = [(int(k), round(rng.uniform(0.5, 12.0), 2))
+                     for k in sorted(picks)]
+        if r.kind in ('troe', 'lind', 'sri'):
+            A, b, Ea = r.arr
+            logA0 = math.log10(A) + rng.uniform(2.0, 6.0)
+            r.low = (float('%.6E' % 10.0 ** logA0),
+                     round(rng.uniform(-2.0, 3.0), 3),
+                     round(rng.uniform(0.0, 60.0) * 1000.0, 3))
+        if r.kind == 'troe':
+            r.troe = (float('%.4f' % rng.uniform(0.2, 0.9)),
+                      float('%.6g' % rng.uniform(50.0, 500.0)),
+                      float('%.6g' % rng.uniform(500.0, 3000.0)),
+                      float('%.6g' % rng.uniform(3000.0, 8000.0)))
+        if r.kind == 'sri':
+            r.sri = (float('%.6g' % rng.uniform(0.1, 10.0)),
+                     float('%.6g' % rng.uniform(100.0, 2000.0)),
+                     float('%.6g' % rng.uniform(100.0, 2000.0)))
+        rxns.append(r)
+
+    # thermo
+    thermo = ['THERMO\n', '   300.000  1000.000  5000.000\n']
+    for i, comp in enumerate(comps):
+        lo, hi = _fit_nasa(rng)
+        counts = [(el, n) for el, n in zip(_ELEMS, comp) if n]
+        thermo.append(_thermo_entry(names[i], counts, lo, hi))
+    # bath gas: drawn like the others from its own stream; it only ever
+    # acts as a collider, so its thermo never enters an equilibrium constant
+    lo, hi = _fit_nasa(np.random.default_rng(cfg.seed + 7919))
+    thermo.append(_thermo_entry('N2', [('N', 2)], lo, hi))
+    thermo.append('END\n')
+
+    # mechanism text
+    mech = ['ELEMENTS\nC H O N\nEND\n', 'SPECIES\n']
+    for k in range(0, nsp, 8):
+        mech.append(' '.join(names[k:k + 8]) + '\n')
+    mech.append('END\n')
+    mech.append('REACTIONS\n')
+    for r in rxns:
+        lhs = _side_text(r.reac, names)
+        rhs = _side_text(r.prod, names)
+        if r.kind == 'thd':
+            lhs += ' + M'
+            rhs += ' + M'
+        elif r.kind in ('troe', 'lind', 'sri'):
+            lhs += ' (+M)'
+            rhs += ' (+M)'
+        arrow = '<=>' if r.rev else '=>'
+        A, b, Ea = r.arr
+        mech.append('{} {} {}   {:.6E} {:.3f} {:.3f}\n'.format(
+            lhs, arrow, rhs, A, b, Ea))
+        if r.low:
+            mech.append('    LOW / {:.6E} {:.3f} {:.3f} /\n'.format(*r.low))
+        if r.troe:
+            mech.append('    TROE / {} {} {} {} /\n'.format(
+                *('%.6g' % x for x in r.troe)))
+        if r.sri:
+            mech.append('    SRI / {} {} {} /\n'.format(
+                *('%.6g' % x for x in r.sri)))
+        if r.eff:
+            mech.append('    ' + ' '.join('{}/{:.2f}/'.format(names[k], e)
+                                          for k, e in r.eff) + '\n')
+    mech.append('END\n')
+    return ''.join(mech), ''.join(thermo)
+
+
+def write_mechanism(cfg, directory):
+    """Write ``<name>.dat`` and ``<name>_therm.dat``; return both paths."""
+    os.makedirs(directory, exist_ok=True)
+    mech, thermo = generate_mechanism(cfg)
+    mpath = os.path.join(directory, cfg.name + '.dat')
+    tpath = os.path.join(directory, cfg.name + '_therm.dat')
+    with open(mpath, 'w') as fh:
+        fh.write(mech)
+    with open(tpath, 'w') as fh:
+        fh.write(thermo)
+    return mpath, tpath

@@ -1,0 +1,129 @@
+/*
+ * C ABI of a create_rate_subs_amd mechanism library (libcrs_<mech>.so).
+ *
+ * One shared library is emitted and built per mechanism, the way the
+ * reference emits one set of C/CUDA subroutines per mechanism.  Every
+ * library exports the symbols below with plain C types only; HIP streams
+ * travel as `void *` (a hipStream_t, NULL for the default stream).
+ *
+ * Data layout (device memory, FP64, structure of arrays): T[n] in K,
+ * P[n] in Pa, and every per-species / per-reaction quantity as rows of a
+ * [rows][ld] array, element (row r, state i) at r*ld + i, ld >= n.
+ * Species are in the library's order (bath gas last).  Mass-fraction input
+ * Y has NSP rows of which rows 0..NSP-2 are read; the bath-gas fraction is
+ * 1 - sum(others), as in the reference's eval_conc.
+ *
+ * Errors: every entry point returns CRS_OK (0) or a nonzero CRS_ERR_* code
+ * and never aborts; crs_last_error() gives a message for the calling
+ * thread.  Work is enqueued asynchronously on the given stream; results
+ * are ready when the stream is synchronised.  The library holds no mutable
+ * state besides per-thread error text and per-device constant-table
+ * addresses, so concurrent calls from several threads are safe.
+ */
+#ifndef CRS_HIP_H
+#define CRS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRS_OK 0
+#define CRS_ERR_ARG 1         /* bad pointer, size, leading dimension or mode */
+#define CRS_ERR_HIP 2         /* a HIP runtime call or kernel launch failed */
+#define CRS_ERR_UNSUPPORTED 3 /* mode not compiled into this library */
+
+/* Precision modes of crs_eval_wdot. */
+#define CRS_MODE_FP64 0  /* FP64 throughout, the drop-in path */
+#define CRS_MODE_MIXED 1 /* FP32 rate constants, FP64 Kc / net rate / sums */
+
+typedef struct crs_mech_info {
+    int32_t nsp;  /* species (bath gas last) */
+    int32_t nr;   /* reactions */
+    int32_t nrev; /* reversible reactions (rows of rev_rates) */
+    int32_t npd;  /* third-body + falloff reactions (rows of pres_mod) */
+    int32_t modes; /* bit (1 << mode) set for every mode compiled in */
+    const char *name;
+} crs_mech_info;
+
+/* Sizes of this library's mechanism.  Host-only; no device work. */
+int crs_get_mech_info(crs_mech_info *out);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *crs_last_error(void);
+
+/*
+ * Batched eval_conc: per state, the implied bath-gas fraction y_N, the mean
+ * molecular weight mw_avg, the density rho and concentrations conc[NSP].
+ * Replaces `void eval_conc(const double T, const double pres,
+ * const double *y, double *y_N, double *mw_avg, double *rho, double *conc)`
+ * emitted by rate_subs.py:1292-1391 (prototype rate_subs.py:1267-1282).
+ * y_N, mw_avg, rho: [n]; conc: [NSP][ld].
+ */
+int crs_eval_conc(const double *T, const double *P, const double *Y,
+                  double *y_N, double *mw_avg, double *rho, double *conc,
+                  int64_t n, int64_t ld, void *stream);
+
+/*
+ * Batched eval_rxn_rates: forward rates fwd[NR][ld] and reverse rates of
+ * the reversible reactions rev[NREV][ld] (compact order, as the reference).
+ * Replaces `void eval_rxn_rates(const double T, const double pres,
+ * const double *C, double *fwd_rxn_rates, double *rev_rxn_rates)`
+ * emitted by rate_subs.py:226-704 (prototype rate_subs.py:259-272).
+ */
+int crs_eval_rxn_rates(const double *T, const double *P, const double *C,
+                       double *fwd, double *rev, int64_t n, int64_t ld,
+                       void *stream);
+
+/*
+ * Batched get_rxn_pres_mod: third-body concentrations and falloff factors
+ * pres_mod[NPD][ld].  Replaces `void get_rxn_pres_mod(const double T,
+ * const double pres, const double *C, double *pres_mod)` emitted by
+ * rate_subs.py:707-1073 (prototype rate_subs.py:274-277).  With NPD == 0
+ * it validates its arguments and does nothing, as the reference emits no
+ * such function then.
+ */
+int crs_get_rxn_pres_mod(const double *T, const double *P, const double *C,
+                         double *pres_mod, int64_t n, int64_t ld,
+                         void *stream);
+
+/*
+ * Batched eval_spec_rates: sp_rates[NSP-1][ld] and the bath gas' rate
+ * dy_N[n].  Replaces `void eval_spec_rates(const double *fwd_rates,
+ * const double *rev_rates, const double *pres_mod, double *sp_rates,
+ * double *dy_N)` emitted by rate_subs.py:1076-1235 (prototype
+ * rate_subs.py:268-269).  Passing dy_N = sp_rates + (NSP-1)*ld gives the
+ * [NSP][ld] layout of crs_eval_wdot.  rev / pres_mod may be NULL when the
+ * mechanism has no such reactions.
+ */
+int crs_eval_spec_rates(const double *fwd, const double *rev,
+                        const double *pres_mod, double *sp_rates,
+                        double *dy_N, int64_t n, int64_t ld, void *stream);
+
+/*
+ * Fused conc -> rates -> pres_mod -> species rates: wdot[NSP][ld], with
+ * row NSP-1 the bath gas (the reference's dy_N).  The per-state call chain
+ * it replaces is the reference's dydt body, rate_subs.py:1845-1880.
+ * mode: CRS_MODE_FP64 or CRS_MODE_MIXED.
+ */
+int crs_eval_wdot(const double *T, const double *P, const double *Y,
+                  double *wdot, int64_t n, int64_t ld, int mode,
+                  void *stream);
+
+/*
+ * crs_eval_wdot on a subset: for t < m, state idx[t] of the [.][ld]
+ * arrays is evaluated and written to column idx[t] of wdot; other columns
+ * are untouched.  Used to re-evaluate gated states exactly.  Indices must
+ * lie in [0, ld); idx is a device array.  No counterpart in the reference
+ * (its states are all evaluated every call).
+ */
+int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
+                         double *wdot, const int32_t *idx, int64_t m,
+                         int64_t ld, int mode, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRS_HIP_H */

@@ -1,0 +1,437 @@
+"""C emitter for the CPU oracle (test infrastructure, never shipped).
+
+Restates the C branch of the reference's generator for the four functions
+on the wdot path, formula for formula and constant format for constant
+format, so that compiling its output gives the subroutines the reference
+would have generated for the same mechanism:
+
+* ``eval_conc``          rate_subs.py:1292-1391 (chem_utils.c)
+* ``eval_rxn_rates``     rate_subs.py:226-704, Arrhenius text :20-135
+* ``get_rxn_pres_mod``   rate_subs.py:707-1073
+* ``eval_spec_rates``    rate_subs.py:1076-1235
+* ``header.h``           create_rate_subs.py:332-427 (the parts the four
+  functions need)
+
+The reference is Python 2; every float it prints with ``str()`` or an empty
+format spec is spelled here with ``utils.py2_str`` (12 significant digits),
+and every other literal with the same ``{:.16e}``/``{:.8e}``/``{:.6}``
+format the reference uses.  It works directly on the reference's data model
+(``SpecInfo``/``ReacInfo`` attributes), not on the HIP emitter's tables, so
+that the two code paths share nothing after parsing.
+
+PLOG and Chebyshev reactions are not on the measured path and raise
+``NotImplementedError``.
+"""
+
+import math
+import os
+
+from create_rate_subs_amd import chem
+from create_rate_subs_amd.utils import (py2_str, fmt_e16, fmt_e8, fmt_g6,
+                                        fmt_w3, is_integer, get_nu)
+
+__all__ = ['arrhenius_text', 'emit_c', 'mech_dims']
+
+
+def arrhenius_text(A, b, E):
+    """Right-hand side for a rate constant, as rate_subs.py:20-135 spells it."""
+    if A > 0:
+        logA = math.log(A)
+        if not E:
+            if not b:
+                return py2_str(A)
+            if isinstance(b, int):
+                return py2_str(A) + ' * T' * max(b, 0)
+            s = 'exp({}'.format(fmt_e16(logA))
+            s += (' + ' + py2_str(b)) if b > 0 else (' - ' + py2_str(abs(b)))
+            return s + ' * logT)'
+        if not b:
+            return 'exp({} - ({} / T))'.format(fmt_e16(logA), fmt_e16(E))
+        s = 'exp({}'.format(fmt_e16(logA))
+        s += (' + ' + py2_str(b)) if b > 0 else (' - ' + py2_str(abs(b)))
+        return s + ' * logT - ({} / T))'.format(fmt_e16(E))
+    if A < 0:
+        if not E:
+            if not b:
+                return py2_str(A)
+            if is_integer(b):
+                return py2_str(A) + ' * T' * max(int(b), 0)
+            s = '{} * exp('.format(fmt_e16(A))
+            s += py2_str(b) if b > 0 else '-' + py2_str(abs(b))
+            return s + ' * logT)'
+        if not b:
+            return '{} * exp(-({} / T))'.format(fmt_e16(A), fmt_e16(E))
+        s = '{} * exp('.format(fmt_e16(A))
+        s += py2_str(b) if b > 0 else '-' + py2_str(abs(b))
+        return s + ' * logT - ({} / T))'.format(fmt_e16(E))
+    raise NotImplementedError('zero pre-exponential factor')
+
+
+def _conc_product(species, nus, trailing_space):
+    """Concentration product for a rate line (rate_subs.py:487-499, :667-680)."""
+    s = ''
+    for isp, nu in zip(species, nus):
+        if is_integer(nu):
+            s += 'C[{}] * '.format(isp) * int(nu)
+        else:
+            s += 'pow(C[{}], {}) *'.format(isp, py2_str(nu))
+            if trailing_space:
+                s += ' '
+    return s
+
+
+def _gibbs_terms(sp, scale):
+    """Lumped Kc coefficients of one species (rate_subs.py:410-431).
+
+    ``scale`` is the signed stoichiometric factor; the seven numbers
+    multiply [1, logT, T, T^2, T^3, T^4, -1/T] in the emitted polynomial.
+    """
+    def one(a):
+        raw = [a[6] - a[0], a[0] - 1.0, a[1] / 2.0, a[2] / 6.0,
+               a[3] / 12.0, a[4] / 20.0, a[5]]
+        return [x * scale for x in raw]
+    return one(sp.lo), one(sp.hi)
+
+
+def _kc_lines(rxn, specs):
+    groups = {}
+    order = []
+    sum_nu = 0
+
+    def add(sp, scale):
+        lo, hi = _gibbs_terms(sp, scale)
+        key = sp.Trange[1]
+        if key not in groups:
+            groups[key] = (lo, hi)
+            order.append(key)
+        else:
+            plo, phi = groups[key]
+            groups[key] = ([lo[i] + plo[i] for i in range(7)],
+                           [hi[i] + phi[i] for i in range(7)])
+
+    for i, isp in enumerate(rxn.prod):
+        if isp in rxn.reac:
+            nu = rxn.prod_nu[i] - rxn.reac_nu[rxn.reac.index(isp)]
+        else:
+            nu = rxn.prod_nu[i]
+        if nu == 0:
+            continue
+        sum_nu += nu
+        add(specs[isp], nu * 1.0)
+    for i, isp in enumerate(rxn.reac):
+        if isp in rxn.prod:
+            continue
+        nu = rxn.reac_nu[i]
+        sum_nu -= nu
+        add(specs[isp], nu * -1.0)
+
+    def poly(c):
+        return ('({} + {} * logT + T * ({} + T * ({} + T * ({} + {} * T))) - '
+                '{} / T)'.format(*[fmt_e16(x) for x in c]))
+
+    out = ''
+    first = True
+    for t_mid in order:
+        lo, hi = groups[t_mid]
+        op = '    Kc = ' if first else '    Kc += '
+        out += '  if (T <= {}) {{\n'.format(py2_str(t_mid))
+        out += op + poly(lo) + ';\n'
+        out += '  } else {\n'
+        out += op + poly(hi) + ';\n'
+        out += '  }\n\n'
+        first = False
+    out += '  Kc = {} * exp(Kc);\n'.format(fmt_e16((chem.PA / chem.RU) ** sum_nu))
+    return out
+
+
+def _write_rxn_rates(specs, reacs):
+    rev_reacs = [i for i, r in enumerate(reacs) if r.rev]
+    body = ['#include "rates.h"\n',
+            'void eval_rxn_rates (const double T, const double pres, '
+            'const double * C, double * fwd_rxn_rates, '
+            'double * rev_rxn_rates) {\n',
+            '  double logT = log(T);\n', '\n']
+    if rev_reacs and any(not r.rev_par for r in reacs):
+        body.append('  double kf;\n  double Kc;\n')
+    body.append('\n')
+    for j, rxn in enumerate(reacs):
+        if rxn.cheb or rxn.plog:
+            raise NotImplementedError('PLOG/Chebyshev reactions')
+        body.append('  //rxn {}\n'.format(j))
+        uses_kf = rxn.rev and not rxn.rev_par
+        if uses_kf:
+            body.append('  kf = ' + arrhenius_text(rxn.A, rxn.b, rxn.E) +
+                        ';\n')
+        line = '  fwd_rxn_rates[{}] = '.format(j)
+        line += _conc_product(rxn.reac, rxn.reac_nu, trailing_space=False)
+        line += 'kf' if uses_kf else arrhenius_text(rxn.A, rxn.b, rxn.E)
+        body.append(line + ';\n')
+        if rxn.rev:
+            if not rxn.rev_par:
+                body.append(_kc_lines(rxn, specs))
+            line = '  rev_rxn_rates[{}] = '.format(rev_reacs.index(j))
+            line += _conc_product(rxn.prod, rxn.prod_nu, trailing_space=True)
+            if rxn.rev_par:
+                line += arrhenius_text(*rxn.rev_par[0:3])
+            else:
+                line += 'kf / Kc'
+            body.append(line + ';\n')
+            body.append('\n')
+    body.append('} // end eval_rxn_rates\n\n')
+    return ''.join(body)
+
+
+def _efficiency_terms(rxn):
+    s = ''
+    for isp, eff in rxn.thd_body_eff:
+        if eff == 1.0:
+            continue
+        if eff > 1.0:
+            s += ' + {}'.format(py2_str(eff - 1.0))
+        else:
+            s += ' - {}'.format(py2_str(1.0 - eff))
+        s += ' * C[{}]'.format(isp)
+    return s
+
+
+def _write_pres_mod(specs, reacs):
+    thd_flag = any(r.thd_body for r in reacs)
+    pdep_flag = any(r.pdep and not r.thd_body for r in reacs)
+    troe_flag = any(r.pdep and r.troe for r in reacs if not r.thd_body)
+    sri_flag = any(r.pdep and r.sri for r in reacs if not r.thd_body)
+    out = ['#include <math.h>\n#include "header.h"\n\n',
+           'void get_rxn_pres_mod (const double T, const double pres, '
+           'const double * C, double * pres_mod) {\n']
+    if thd_flag or pdep_flag:
+        out.append('  // third body variable declaration\n  double thd;\n\n')
+    if pdep_flag:
+        out.append('  // pressure dependence variable declarations\n'
+                   '  double k0;\n  double kinf;\n  double Pr;\n\n')
+        if troe_flag:
+            out.append('  // troe variable declarations\n'
+                       '  double logFcent;\n  double A;\n  double B;\n\n')
+        if sri_flag:
+            out.append('  // sri variable declarations\n  double X;\n\n')
+    out.append('  double logT = log(T);\n'
+               '  double m = pres / ({} * T);\n'.format(fmt_e8(chem.RU)))
+    out.append('\n')
+    p = 0
+    for j, rxn in enumerate(reacs):
+        if not (rxn.pdep or rxn.thd_body):
+            continue
+        out.append('  // reaction {};\n'.format(j))
+        if rxn.thd_body:
+            out.append('  pres_mod[{}] = m{};\n'.format(
+                p, _efficiency_terms(rxn)))
+        if rxn.pdep:
+            if rxn.pdep_sp == '':
+                out.append('  thd = m{};\n'.format(_efficiency_terms(rxn)))
+            low = rxn.low if rxn.low else [rxn.A, rxn.b, rxn.E]
+            high = rxn.high if rxn.high else [rxn.A, rxn.b, rxn.E]
+            out.append('  k0 = ' + arrhenius_text(*low[0:3]) + ';\n')
+            out.append('  kinf = ' + arrhenius_text(*high[0:3]) + ';\n')
+            if rxn.pdep_sp != '':
+                out.append('  Pr = k0 * C[{}] / kinf;\n'.format(rxn.pdep_sp))
+            else:
+                out.append('  Pr = k0 * thd / kinf;\n')
+            simple = False
+            if rxn.troe:
+                tp = rxn.troe_par
+                s = '  logFcent = log10( fmax({} * '.format(fmt_e8(1.0 - tp[0]))
+                s += ('exp(-T / {})'.format(fmt_e8(tp[1])) if tp[1] > 0.0
+                      else 'exp(T / {})'.format(fmt_e8(abs(tp[1]))))
+                s += ' + {} * '.format(fmt_e8(tp[0]))
+                s += ('exp(-T / {})'.format(fmt_e8(tp[2])) if tp[2] > 0.0
+                      else 'exp(T / {})'.format(fmt_e8(abs(tp[2]))))
+                if len(tp) == 4 and tp[3] != 0.0:
+                    s += ' + '
+                    s += ('exp(-{} / T)'.format(fmt_e8(tp[3])) if tp[3] > 0.0
+                          else 'exp({} / T)'.format(fmt_e8(abs(tp[3]))))
+                s += ', 1.0e-300));\n'
+                out.append(s)
+                out.append('  A = log10(fmax(Pr, 1.0e-300)) - 0.67 * '
+                           'logFcent - 0.4;\n')
+                out.append('  B = 0.806 - 1.1762 * logFcent - 0.14 * '
+                           'log10(fmax(Pr, 1.0e-300));\n')
+                line = ('  pres_mod[{}] = pow(10.0, logFcent / '
+                        '(1.0 + A * A / (B * B))) '.format(p))
+            elif rxn.sri:
+                sp_ = rxn.sri_par
+                out.append('  X = 1.0 / (1.0 + log10(fmax(Pr, 1.0e-300)) * '
+                           'log10(fmax(Pr, 1.0e-300)));\n')
+                line = '  pres_mod[{}] = pow({} * '.format(p, fmt_g6(sp_[0]))
+                line += ('exp(-{} / T)'.format(fmt_g6(sp_[1])) if sp_[1] > 0.0
+                         else 'exp({} / T)'.format(fmt_g6(abs(sp_[1]))))
+                line += (' + exp(-T / {}), X) '.format(fmt_g6(sp_[2]))
+                         if sp_[2] > 0.0 else
+                         ' + exp(T / {}), X) '.format(fmt_g6(abs(sp_[2]))))
+                if len(sp_) == 5 and sp_[3] != 1.0 and sp_[4] != 0.0:
+                    line += '* {} * pow(T, {}) '.format(fmt_e8(sp_[3]),
+                                                        fmt_g6(sp_[4]))
+            else:
+                simple = True
+                line = '  pres_mod[{}] = '.format(p)
+                if rxn.low:
+                    line += ' Pr / (1.0 + Pr)'
+                elif rxn.high:
+                    line += '1.0 / (1.0 + Pr)'
+            if not simple:
+                if rxn.low:
+                    line += '* Pr / (1.0 + Pr)'
+                elif rxn.high:
+                    line += '/ (1.0 + Pr)'
+            out.append(line + ';\n')
+        out.append('\n')
+        p += 1
+    out.append('} // end get_rxn_pres_mod\n\n')
+    return ''.join(out)
+
+
+def _write_spec_rates(specs, reacs, spec_order):
+    rev_reacs = [i for i, r in enumerate(reacs) if r.rev]
+    pdep_reacs = [i for i, r in enumerate(reacs) if r.thd_body or r.pdep]
+    nsp = len(specs)
+    out = ['#include "header.h"\n\n',
+           'void eval_spec_rates (const double * fwd_rates, '
+           'const double * rev_rates, const double * pres_mod, '
+           'double * sp_rates, double * dy_N) {\n']
+
+    def target(k):
+        return '  (*dy_N)' if k + 1 == nsp else '  sp_rates[{}]'.format(k)
+
+    seen = [False] * nsp
+    for j, rxn in enumerate(reacs):
+        out.append('  //rxn {}\n'.format(j))
+        # each species of a reaction writes a different accumulator, so the
+        # order within a reaction does not change any value
+        for k in sorted(set(rxn.reac + rxn.prod)):
+            nu = get_nu(k, rxn)
+            if nu == 0.0:
+                continue
+            out.append('  //sp {}\n'.format(spec_order[k]))
+            sign = '-' if nu < 0 else '+'
+            line = target(k) + ' {}= '.format(sign if seen[k] else '')
+            if not seen[k] and nu < 0:
+                line += sign
+            anu = abs(nu)
+            if anu != 1.0:
+                if is_integer(anu):
+                    line += '{} * '.format(py2_str(float(anu)))
+                else:
+                    line += '{} * '.format(fmt_w3(anu))
+            if rxn.rev:
+                term = '(fwd_rates[{}] - rev_rates[{}])'.format(
+                    j, rev_reacs.index(j))
+            else:
+                term = 'fwd_rates[{}]'.format(j)
+            if rxn.thd_body or rxn.pdep:
+                term += ' * pres_mod[{}]'.format(pdep_reacs.index(j))
+            seen[k] = True
+            out.append(line + term + ';\n')
+        out.append('\n')
+    for k in range(nsp):
+        if not seen[k]:
+            out.append('  //sp {}\n'.format(spec_order[k]))
+            out.append(target(k) + ' = 0.0;\n')
+    out.append('} // end eval_spec_rates\n\n')
+    return ''.join(out)
+
+
+def _write_eval_conc(specs):
+    nsp = len(specs)
+    out = ['#include "header.h"\n\n',
+           'void eval_conc (const double T, const double pres, '
+           'const double * y, double * y_N, double * mw_avg, '
+           'double * rho, double * conc) {\n\n',
+           '  // mass fraction of final species\n']
+    line = '  *y_N = 1.0 - ('
+    for k in range(nsp - 1):
+        if len(line) > 70:
+            out.append(line + '\n')
+            line = '               '
+        if k:
+            line += ' + '
+        line += 'y[{}]'.format(k)
+    out.append(line + ');\n')
+    line = '  *mw_avg = '
+    for k in range(nsp - 1):
+        if len(line) > 70:
+            out.append(line + '\n')
+            line = '     '
+        if k:
+            line += ' + '
+        line += '(y[{}] * {})'.format(k, fmt_e16(1.0 / specs[k].mw))
+    line += ' + ((*y_N) * {});\n'.format(fmt_e16(1.0 / specs[-1].mw))
+    out.append(line)
+    out.append('  *mw_avg = 1.0 / *mw_avg;\n')
+    out.append('  // mass-averaged density\n')
+    out.append('  *rho = pres * (*mw_avg) / ({} * T);\n'.format(fmt_e8(chem.RU)))
+    for k in range(nsp - 1):
+        out.append('  conc[{0}] = (*rho) * y[{0}] * {1};\n'.format(
+            k, fmt_e16(1.0 / specs[k].mw)))
+    out.append('  conc[{}] = (*rho) * (*y_N) * {};\n\n'.format(
+        nsp - 1, fmt_e16(1.0 / specs[-1].mw)))
+    out.append('} // end eval_conc\n\n')
+    return ''.join(out)
+
+
+def _write_headers(specs, reacs):
+    nsp = len(specs)
+    pdep = any(r.thd_body or r.pdep for r in reacs)
+    header = ('#include <stdlib.h>\n#include <stdio.h>\n#include <math.h>\n'
+              '#include <string.h>\n#include <stdbool.h>\n\n'
+              '#define DOUBLE\n#define Real double\n\n'
+              '/** Number of species.\n')
+    for k, sp in enumerate(specs):
+        header += ' * {} {}\n'.format(k, sp.name)
+    header += ' */\n#define NSP {}\n#define NN {}\n\n'.format(nsp, nsp + 1)
+    rates = ('#ifndef RATES_HEAD\n#define RATES_HEAD\n\n#include "header.h"\n\n'
+             'void eval_rxn_rates (const double, const double, const double*, '
+             'double*, double*);\n'
+             'void eval_spec_rates (const double*, const double*, '
+             'const double*, double*, double*);\n')
+    if pdep:
+        rates += ('void get_rxn_pres_mod (const double, const double, '
+                  'const double*, double*);\n')
+    rates += '\n#endif\n'
+    chem_h = ('#ifndef CHEM_UTILS_HEAD\n#define CHEM_UTILS_HEAD\n\n'
+              '#include "header.h"\n\n'
+              'void eval_conc (const double, const double, const double*, '
+              'double*, double*, double*, double*);\n\n#endif\n')
+    return header, rates, chem_h
+
+
+def mech_dims(specs, reacs):
+    """(NSP, NR, NREV, NPD) of a mechanism."""
+    return (len(specs), len(reacs), sum(1 for r in reacs if r.rev),
+            sum(1 for r in reacs if r.thd_body or r.pdep))
+
+
+def emit_c(specs, reacs, outdir, spec_order=None):
+    """Write the oracle's C sources for the mechanism into ``outdir``.
+
+    ``specs``/``reacs`` are in final order with integer species indices
+    (after the bath-gas reordering).  ``spec_order[k]`` is the original
+    index of species k, used only in comments.  Returns the list of C files.
+    """
+    if spec_order is None:
+        spec_order = list(range(len(specs)))
+    os.makedirs(outdir, exist_ok=True)
+    header, rates_h, chem_h = _write_headers(specs, reacs)
+    nsp, nr, nrev, npd = mech_dims(specs, reacs)
+    dims = ('#ifndef MECH_DIMS_HEAD\n#define MECH_DIMS_HEAD\n'
+            '#define MECH_NR {}\n#define MECH_NREV {}\n#define MECH_NPD {}\n'
+            '#endif\n'.format(nr, nrev, npd))
+    files = {
+        'header.h': header, 'rates.h': rates_h, 'chem_utils.h': chem_h,
+        'mech_dims.h': dims,
+        'chem_utils.c': _write_eval_conc(specs),
+        'rxn_rates.c': _write_rxn_rates(specs, reacs),
+        'spec_rates.c': _write_spec_rates(specs, reacs, spec_order),
+    }
+    if npd:
+        files['rxn_rates_pres_mod.c'] = _write_pres_mod(specs, reacs)
+    for name, text in files.items():
+        with open(os.path.join(outdir, name), 'w') as fh:
+            fh.write(text)
+    return [os.path.join(outdir, n) for n in files if n.endswith('.c')]

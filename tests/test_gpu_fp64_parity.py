@@ -1,0 +1,124 @@
+"""FP64 HIP path against the C oracle (GPU).
+
+Stage kernels are compared on the same inputs: eval_conc and
+eval_spec_rates must be bit-identical to the oracle (same operations, no
+contraction, IEEE division on both sides); eval_rxn_rates and
+get_rxn_pres_mod may differ by the last bits of libm's exp/log (glibc on
+the host, the ROCm device library on the GPU), bounded at 1e-12 pointwise.
+The fused kernel is scored with the conditioning-aware relative error of
+create_rate_subs_amd.metrics against the oracle at <= 1e-12.
+"""
+
+import numpy as np
+import pytest
+
+from create_rate_subs_amd import metrics, states
+
+from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+MECHS = ['mech13', 'mech53', 'mech111']
+# tau of the conditioned error: below tau * S_k, errors count relative to
+# tau * S_k.  1e-3 * S_k * 1e-12 = 1e-15 S_k, a few ulps of the terms.
+TAU = 1e-3
+
+
+def _states(name, n, seed):
+    specs, _r, _f = mechanism(name)
+    return states.random_states(len(specs), n, seed)
+
+
+@pytest.mark.parametrize('name', MECHS)
+@pytest.mark.parametrize('n', [1, 257, 4096])
+def test_eval_conc_bit_exact(name, n):
+    T, P, Y = _states(name, n, 11)
+    ref = oracle(name).eval_stages(T, P, Y)
+    lib = hip_lib(name)
+    out = lib.eval_conc(*to_dev(T, P, Y))
+    for key in ('y_N', 'mw', 'rho', 'C'):
+        np.testing.assert_array_equal(to_host(out[key]), ref[key], err_msg=key)
+
+
+@pytest.mark.parametrize('name', MECHS)
+def test_rxn_rates_and_pres_mod(name):
+    T, P, Y = _states(name, 4096, 12)
+    ref = oracle(name).eval_stages(T, P, Y)
+    lib = hip_lib(name)
+    Td, Pd, Cd = to_dev(T, P, ref['C'])
+    fwd, rev = lib.eval_rxn_rates(Td, Pd, Cd)
+    pm = lib.get_rxn_pres_mod(Td, Pd, Cd)
+    for key, got in (('fwd', fwd), ('rev', rev), ('pm', pm)):
+        r = ref[key]
+        g = to_host(got)
+        rel = np.abs(g - r) / np.maximum(np.abs(r), 1e-300)
+        assert rel.max() <= 1e-12, (key, rel.max())
+
+
+@pytest.mark.parametrize('name', MECHS)
+def test_spec_rates_bit_exact(name):
+    T, P, Y = _states(name, 4096, 13)
+    o = oracle(name)
+    ref = o.eval_stages(T, P, Y)
+    lib = hip_lib(name)
+    fwd, rev, pm = to_dev(ref['fwd'], ref['rev'] if lib.nrev else
+                          np.zeros((1, 4096)),
+                          ref['pm'] if lib.npd else np.zeros((1, 4096)))
+    W = lib.eval_spec_rates(fwd, rev[:lib.nrev], pm[:lib.npd])
+    np.testing.assert_array_equal(to_host(W), ref['wdot'])
+
+
+@pytest.mark.parametrize('name', MECHS)
+@pytest.mark.parametrize('n', [1, 100, 4096, 65536])
+def test_wdot_fp64_matches_oracle(name, n):
+    specs, reacs, _ = mechanism(name)
+    T, P, Y = _states(name, n, 14)
+    ref = oracle(name).eval_stages(T, P, Y)
+    lib = hip_lib(name)
+    W = to_host(lib.eval_wdot(*to_dev(T, P, Y)))
+    S = metrics.gross_scale(specs, reacs, ref['fwd'], ref['rev'], ref['pm'])
+    err = metrics.rel_err(W, ref['wdot'], S, TAU)
+    s = metrics.summarize(err)
+    print(name, n, s)
+    assert s['max'] <= 1e-12, s
+
+
+@pytest.mark.parametrize('name', ['mech53'])
+def test_wdot_subset(name):
+    import torch
+    T, P, Y = _states(name, 5000, 15)
+    lib = hip_lib(name)
+    Td, Pd, Yd = to_dev(T, P, Y)
+    full = lib.eval_wdot(Td, Pd, Yd)
+    idx = torch.tensor([4999, 0, 17, 2500, 17], dtype=torch.int32,
+                       device='cuda')
+    W = torch.full_like(full, np.nan)
+    lib.eval_wdot_subset(Td, Pd, Yd, W, idx)
+    Wh, Fh = to_host(W), to_host(full)
+    sel = [0, 17, 2500, 4999]
+    np.testing.assert_array_equal(Wh[:, sel], Fh[:, sel])
+    rest = np.setdiff1d(np.arange(5000), sel)
+    assert np.isnan(Wh[:, rest]).all()
+
+
+def test_wdot_empty_and_bad_args():
+    import ctypes
+    import torch
+    lib = hip_lib('mech13')
+    e = torch.empty(0, dtype=torch.float64, device='cuda')
+    Y = torch.empty((lib.nsp, 0), dtype=torch.float64, device='cuda')
+    W = lib.eval_wdot(e, e, Y)
+    assert W.shape == (lib.nsp, 0)
+    raw = lib.lib
+    null = ctypes.c_void_p(0)
+    # unknown mode, n < 0, ld < n, NULL pointer with work: all rejected
+    T = torch.ones(4, dtype=torch.float64, device='cuda')
+    Y4 = torch.ones((lib.nsp, 4), dtype=torch.float64, device='cuda')
+    p = [ctypes.c_void_p(t.data_ptr()) for t in (T, T, Y4, Y4)]
+    assert raw.crs_eval_wdot(*p, 4, 4, 7, null) == 1
+    assert b'mode' in raw.crs_last_error()
+    assert raw.crs_eval_wdot(*p, -1, 4, 0, null) == 1
+    assert raw.crs_eval_wdot(*p, 4, 3, 0, null) == 1
+    assert raw.crs_eval_wdot(p[0], null, p[2], p[3], 4, 4, 0, null) == 1
+    assert b'P' in raw.crs_last_error()
+    torch.cuda.synchronize()
