@@ -278,21 +278,33 @@ def _lit(x):
 
 
 class _Consts(object):
-    """Constant table: every mechanism number lives in one __constant__ array."""
+    """Where mechanism numbers live in the kernel text.
 
-    def __init__(self, symbol='crs_K'):
+    ``table``: one ``__constant__`` array read with wave-uniform scalar
+    loads; ``literal``: immediates in the instruction stream.
+    """
+
+    def __init__(self, mode='table', symbol='crs_K'):
         self.vals = []
         self.index = {}
         self.symbol = symbol
+        self.mode = mode
+        # kernels read the table through a restrict pointer argument so
+        # the compiler keeps them as (batched) scalar loads
+        self.ref = 'K'
 
     def __call__(self, x):
+        if self.mode == 'literal':
+            return _lit(x)
         key = float(x).hex()
         if key not in self.index:
             self.index[key] = len(self.vals)
             self.vals.append(float(x))
-        return '{}[{}]'.format(self.symbol, self.index[key])
+        return '{}[{}]'.format(self.ref, self.index[key])
 
     def definition(self):
+        if self.mode == 'literal':
+            return ''
         body = ',\n  '.join(_lit(v) for v in self.vals) or '0.0'
         return ('__constant__ double {}[{}] = {{\n  {}\n}};\n'.format(
             self.symbol, max(len(self.vals), 1), body))
@@ -483,9 +495,289 @@ def _indent(lines, n):
     return ''.join(pad + l + '\n' for l in lines)
 
 
-def emit_hip_source(ir):
+DEFAULT_OPTS = dict(consts='table', sched_barrier=False, early_store=True,
+                    fused='fast')
+
+
+def _last_touch(ir):
+    last = {}
+    for x in ir.rxns:
+        for k, _nu in x.spec:
+            last[k] = x.index
+    return last
+
+
+def _fused_f64_lines(ir, K, opts):
+    """Per-state body of the fused FP64 kernel.
+
+    Reaction by reaction: rate constant, forward and reverse rates,
+    pressure modification and the species updates, in the reference's
+    reaction order so every species sum keeps the reference's term order.
+    A species' rate is stored as soon as its last reaction is done and a
+    scheduling barrier closes every reaction, which bounds the live set to
+    the concentrations plus the species still being accumulated.
+    """
+    nsp = ir.nsp
+    ru = _lit(emitted(fmt_e8(chem.RU)))
+    lines = ['const double T = T_[i];', 'const double P = P_[i];',
+             'double y[{}];'.format(max(nsp - 1, 1))]
+    lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k) for k in range(nsp - 1)]
+    lines += _conc_lines(ir, K)
+    lines += ['const double logT = log(T);',
+              'const double m = P / ({} * T);'.format(ru),
+              '(void)m;',
+              'double w[{}];'.format(nsp)]
+    last = _last_touch(ir)
+    seen = [False] * nsp
+    for k in range(nsp):
+        if k not in last:
+            lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
+    for x in ir.rxns:
+        inner = _rate_lines(x, K)
+        if x.pd_index is not None:
+            inner.append('double pm;')
+            inner += _pres_mod_lines(x, K)
+        inner += _spec_lines(x, seen)
+        lines.append('{  // rxn %d' % x.index)
+        lines += ['    ' + l for l in inner]
+        lines.append('}')
+        if opts.get('early_store'):
+            for k, _nu in x.spec:
+                if last[k] == x.index:
+                    lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
+        if opts.get('sched_barrier'):
+            lines.append('__builtin_amdgcn_sched_barrier(0);')
+    if not opts.get('early_store'):
+        lines += ['W[{k} * ld + i] = w[{k}];'.format(k=k)
+                  for k in range(nsp) if k in last]
+    return lines
+
+
+_BASIS = ('1.0', 'logT', 'T', 'T2', 'T3', 'T4', 'iT')
+
+
+def _fast_arr(a, K, name):
+    """Lines defining rate constant ``name`` (and its exponent ``x_<name>``).
+
+    Returns (lines, exponent_or_None).  The exponent is lnA + b logT - E/T
+    with 1/T precomputed once per state; forms without an exponential
+    (literal A, A*T^n) are kept as the reference prints them.
+    """
+    if not a.has_exp:
+        return ['const double {} = {};'.format(name, _arr_expr(a, K))], None
+    terms = []
+    if a.logA is not None:
+        terms.append(K(a.logA))
+    if a.b is not None:
+        terms.append('{} * logT'.format(K(a.b)))
+    x = ' + '.join(terms)
+    if a.E is not None:
+        x = '{} - {} * iT'.format(x, K(a.E)) if x else '-{} * iT'.format(K(a.E))
+    out = ['const double x_{} = {};'.format(name, x)]
+    if a.pre is not None:
+        out.append('const double {n} = {p} * exp(x_{n});'.format(n=name,
+                                                                  p=K(a.pre)))
+        return out, None
+    out.append('const double {n} = exp(x_{n});'.format(n=name))
+    return out, 'x_' + name
+
+
+def _fast_lnkc(x, K):
+    """ln Kc as a sum over the per-state masked NASA basis (no selects)."""
+    terms = []
+    for g, (tmid, lo, hi) in enumerate(x.kc_groups):
+        for half, c in (('l', lo), ('h', hi)):
+            coef = list(c[0:6]) + [-c[6]]
+            if g == 0:
+                coef[0] = coef[0] + math.log(x.kc_pref)
+            for bi, cv in enumerate(coef):
+                if cv != 0.0:
+                    terms.append('{} * p{}{}_{}'.format(K(cv), half, g, bi))
+    return ' + '.join(terms) if terms else '0.0'
+
+
+def _fast_pm_lines(x, K, kf_x):
+    """Pressure modification of reaction x into 'pm' (fast FP64 form)."""
+    out = []
+    if x.thd:
+        out.append('const double pm = {};'.format(_eff_expr(x, K)))
+        return out
+    if x.pdep_sp is None:
+        out.append('const double thd = {};'.format(_eff_expr(x, K)))
+        coll = 'thd'
+    else:
+        coll = 'C[{}]'.format(x.pdep_sp)
+    # k0 / kinf: whichever limit is not given is the main Arrhenius (kf)
+    if x.low is not None:
+        l0, x0 = _fast_arr(x.low, K, 'k0')
+        out += l0
+        num, xnum = 'k0', x0
+    else:
+        num, xnum = 'kf', kf_x
+    if x.high is not None:
+        l1, x1 = _fast_arr(x.high, K, 'kinf')
+        out += l1
+        den, xden = 'kinf', x1
+    else:
+        den, xden = 'kf', kf_x
+    if xnum is not None and xden is not None:
+        out.append('const double Pr = {} * exp({} - {});'.format(coll, xnum,
+                                                                  xden))
+    else:
+        out.append('const double Pr = {} * {} / {};'.format(num, coll, den))
+    tail = ''
+    if x.falloff_low:
+        tail = ' * Pr / (1.0 + Pr)'
+    elif x.falloff_high:
+        tail = ' / (1.0 + Pr)'
+    if x.falloff == 'troe':
+        t = x.troe
+        e1 = 'exp({}T * {})'.format('-' if t['t3_pos'] else '', K(1.0 / t['t3']))
+        e2 = 'exp({}T * {})'.format('-' if t['t1_pos'] else '', K(1.0 / t['t1']))
+        fc = '{} * {} + {} * {}'.format(K(t['one_minus_a']), e1, K(t['a']), e2)
+        if t['t2'] is not None:
+            fc += ' + exp({}{} * iT)'.format('-' if t['t2_pos'] else '',
+                                             K(t['t2']))
+        out.append('const double logFcent = log10(fmax({}, 1.0e-300));'.format(fc))
+        out.append('const double lPr = log10(fmax(Pr, 1.0e-300));')
+        out.append('const double A = lPr - 0.67 * logFcent - 0.4;')
+        out.append('const double B = 0.806 - 1.1762 * logFcent - 0.14 * lPr;')
+        out.append('const double pm = exp10(logFcent / (1.0 + A * A / (B * B))){};'
+                   .format(tail))
+    elif x.falloff == 'sri':
+        sr = x.sri
+        out.append('const double lPr = log10(fmax(Pr, 1.0e-300));')
+        out.append('const double X = 1.0 / (1.0 + lPr * lPr);')
+        e1 = 'exp({}{} * iT)'.format('-' if sr['b_pos'] else '', K(sr['b']))
+        e2 = 'exp({}T * {})'.format('-' if sr['c_pos'] else '', K(1.0 / sr['c']))
+        base = '{} * {} + {}'.format(K(sr['a']), e1, e2)
+        if sr['a'] > 0.0:
+            f = 'exp(X * log({}))'.format(base)
+        else:
+            f = 'pow({}, X)'.format(base)
+        if sr['d'] is not None:
+            f += ' * {} * exp({} * logT)'.format(K(sr['d']), K(sr['e']))
+        out.append('const double pm = {}{};'.format(f, tail))
+    else:
+        if x.falloff_low:
+            out.append('const double pm = Pr / (1.0 + Pr);')
+        elif x.falloff_high:
+            out.append('const double pm = 1.0 / (1.0 + Pr);')
+        else:
+            out.append('const double pm = 1.0;')
+    return out
+
+
+def _fused_fast_lines(ir, K, opts):
+    """Per-state body of the fused FP64 kernel, arranged for the GPU.
+
+    Same physics and reaction order as the reference, with the arithmetic
+    rearranged so that a state needs fewer and cheaper operations and no
+    per-reaction data-dependent selects:
+
+    * 1/T, T^2..T^4 once per state; E/T becomes E * (1/T);
+    * Kc from the NASA basis [1, lnT, T, T^2, T^3, T^4, 1/T] pre-masked
+      once per state for each T_mid group (lower/upper range), so ln Kc is
+      a fixed sum of products with constant coefficients;
+    * r = prod C'' * exp(ln kf - ln Kc): one exponential instead of
+      exp(Kc) and a division;
+    * Pr = [M] exp(ln k0 - ln kinf) for falloff (k_inf reuses kf);
+    * each species rate is stored as soon as its last reaction is done.
+    """
+    nsp = ir.nsp
+    ru = _lit(emitted(fmt_e8(chem.RU)))
+    lines = ['const double T = T_[i];', 'const double P = P_[i];',
+             'double y[{}];'.format(max(nsp - 1, 1))]
+    lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k) for k in range(nsp - 1)]
+    lines += _conc_lines(ir, K)
+    lines += ['const double logT = log(T);',
+              'const double iT = 1.0 / T;',
+              'const double T2 = T * T;', 'const double T3 = T2 * T;',
+              'const double T4 = T3 * T;',
+              'const double m = P / ({} * T);'.format(ru),
+              '(void)m; (void)T2; (void)T3; (void)T4;']
+    tmids = []
+    for x in ir.rxns:
+        for tm, _lo, _hi in x.kc_groups:
+            if tm not in tmids:
+                tmids.append(tm)
+    # group index of a reaction's g-th Tmid group -> global mask index
+    for gi, tm in enumerate(tmids):
+        lines.append('const bool lo{g} = T <= {tm};'.format(g=gi, tm=_lit(tm)))
+        for bi, bname in enumerate(_BASIS):
+            lines.append('const double pl{g}_{b} = lo{g} ? {v} : 0.0;'.format(
+                g=gi, b=bi, v=bname))
+            lines.append('const double ph{g}_{b} = lo{g} ? 0.0 : {v};'.format(
+                g=gi, b=bi, v=bname))
+    last = _last_touch(ir)
+    seen = [False] * nsp
+    lines.append('double w[{}];'.format(nsp))
+    for k in range(nsp):
+        if k not in last:
+            lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
+    for x in ir.rxns:
+        inner, xf = _fast_arr(x.arr, K, 'kf')
+        inner.append('const double f = {}kf;'.format(_conc_prod(x.reac)))
+        if x.rev:
+            if x.rev_arr is not None:
+                la, _xr = _fast_arr(x.rev_arr, K, 'kr')
+                inner += la
+                inner.append('const double r = {}kr;'.format(
+                    _conc_prod(x.prod)))
+            else:
+                # remap this reaction's Tmid groups onto the global masks
+                groups = []
+                for tm, lo, hi in x.kc_groups:
+                    groups.append((tmids.index(tm), lo, hi))
+                terms = []
+                for n_g, (gi, lo, hi) in enumerate(groups):
+                    for half, c in (('l', lo), ('h', hi)):
+                        coef = list(c[0:6]) + [-c[6]]
+                        if n_g == 0:
+                            coef[0] = coef[0] + math.log(x.kc_pref)
+                        for bi, cv in enumerate(coef):
+                            if cv != 0.0:
+                                terms.append('{} * p{}{}_{}'.format(
+                                    K(cv), half, gi, bi))
+                lnkc = ' + '.join(terms) if terms else '0.0'
+                inner.append('const double lnKc = {};'.format(lnkc))
+                if xf is not None:
+                    inner.append('const double r = {}exp({} - lnKc);'.format(
+                        _conc_prod(x.prod), xf))
+                else:
+                    inner.append('const double r = {}kf * exp(-lnKc);'.format(
+                        _conc_prod(x.prod)))
+        q = '(f - r)' if x.rev else 'f'
+        if x.pd_index is not None:
+            inner += _fast_pm_lines(x, K, xf)
+            q = q + ' * pm'
+        inner.append('const double q = {};'.format(q))
+        for k, nu in x.spec:
+            anu = abs(nu)
+            term = 'q' if anu == 1.0 else '{} * q'.format(
+                _lit(emitted(py2_str(float(anu)))))
+            if seen[k]:
+                inner.append('w[{k}] = w[{k}] {op} {t};'.format(
+                    k=k, op='-' if nu < 0 else '+', t=term))
+            else:
+                inner.append('w[{k}] = {n}{t};'.format(
+                    k=k, n='-' if nu < 0 else '', t=term))
+                seen[k] = True
+        lines.append('{  // rxn %d' % x.index)
+        lines += ['    ' + l for l in inner]
+        lines.append('}')
+        for k, _nu in x.spec:
+            if last[k] == x.index:
+                lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
+        if opts.get('sched_barrier'):
+            lines.append('__builtin_amdgcn_sched_barrier(0);')
+    return lines
+
+
+def emit_hip_source(ir, opts=None):
     """Full HIP source text for mechanism ``ir``."""
-    K = _Consts()
+    opts = dict(DEFAULT_OPTS, **(opts or {}))
+    K = _Consts(opts['consts'])
     nsp, nr, nrev, npd = ir.nsp, ir.nr, ir.nrev, ir.npd
     ru = _lit(emitted(fmt_e8(chem.RU)))
     body = []
@@ -501,6 +793,7 @@ def emit_hip_source(ir):
     lines += ['Cout[{k} * ld + i] = C[{k}];'.format(k=k) for k in range(nsp)]
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_eval_conc(\n'
+        '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ y_N_out,\n'
         '    double *__restrict__ mw_out, double *__restrict__ rho_out,\n'
@@ -524,6 +817,7 @@ def emit_hip_source(ir):
         lines.append('}')
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_eval_rxn_rates(\n'
+        '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ Cin,\n'
         '    double *__restrict__ fwd, double *__restrict__ rev, int64_t n,\n'
         '    int64_t ld)\n{\n' + _indent(lines, 4) + '}\n\n')
@@ -552,6 +846,7 @@ def emit_hip_source(ir):
         lines.append('(void)logT; (void)m;')
         body.append(
             '__global__ __launch_bounds__(crs::kBlock) void k_get_rxn_pres_mod(\n'
+        '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Cin, double *__restrict__ pres_mod,\n'
             '    int64_t n, int64_t ld)\n{\n' + _indent(lines, 4) + '}\n\n')
@@ -579,57 +874,33 @@ def emit_hip_source(ir):
     lines.append('dy_N[i] = w[{}];'.format(nsp - 1))
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_eval_spec_rates(\n'
+        '    const double *__restrict__ K,\n'
         '    const double *__restrict__ fwd, const double *__restrict__ rev,\n'
         '    const double *__restrict__ pres_mod, double *__restrict__ sp_rates,\n'
         '    double *__restrict__ dy_N, int64_t n, int64_t ld)\n{\n'
         '#pragma clang fp contract(off)\n' + _indent(lines, 4) + '}\n\n')
 
     # ---- fused FP64 per-state body ------------------------------------
-    lines = ['const double T = T_[i];', 'const double P = P_[i];',
-             'double y[{}];'.format(max(nsp - 1, 1))]
-    lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k) for k in range(nsp - 1)]
-    lines += _conc_lines(ir, K)
-    lines += ['const double logT = log(T);',
-              'const double m = P / ({} * T);'.format(ru),
-              '(void)m;',
-              'double w[{}];'.format(nsp)]
-    seen = [False] * nsp
-    for x in ir.rxns:
-        inner = _rate_lines(x, K)
-        if x.pd_index is not None:
-            inner.append('double pm;')
-            inner += _pres_mod_lines(x, K)
-        inner += _spec_lines(x, seen)
-        lines.append('{  // rxn %d' % x.index)
-        lines += ['    ' + l for l in inner]
-        lines.append('}')
-    for k in range(nsp):
-        if not seen[k]:
-            lines.append('w[{}] = 0.0;'.format(k))
-    lines += ['W[{k} * ld + i] = w[{k}];'.format(k=k) for k in range(nsp)]
     body.append(
         '__device__ __forceinline__ void wdot_f64_state(\n'
+        '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
-        '    int64_t ld, int64_t i)\n{\n' + _indent(lines, 4) + '}\n\n')
+        '    int64_t ld, int64_t i)\n{\n' +
+        _indent((_fused_fast_lines if opts['fused'] == 'fast' else
+                 _fused_f64_lines)(ir, K, opts), 4) + '}\n\n')
+    # one kernel serves the full batch (idx == nullptr) and subsets
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_wdot_f64(\n'
-        '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
-        '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
-        '    int64_t ld)\n{\n'
-        '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
-        '    if (i >= n) return;\n'
-        '    wdot_f64_state(T_, P_, Y, W, ld, i);\n}\n\n')
-    body.append(
-        '__global__ __launch_bounds__(crs::kBlock) void k_wdot_f64_subset(\n'
+        '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
         '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
         '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
         '    if (t >= m) return;\n'
-        '    const int64_t i = idx[t];\n'
+        '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
         '    if (i < 0 || i >= ld) return;\n'
-        '    wdot_f64_state(T_, P_, Y, W, ld, i);\n}\n\n')
+        '    wdot_f64_state(K, T_, P_, Y, W, ld, i);\n}\n\n')
 
     header = (
         '// Generated by create_rate_subs_amd.hip_emitter for mechanism '
@@ -649,8 +920,29 @@ def emit_hip_source(ir):
 def _abi_text(ir):
     npd = ir.npd
     pm_launch = ('    k_get_rxn_pres_mod<<<crs::grid_for(n), crs::kBlock, 0, s>>>'
-                 '(T, P, C, pres_mod, n, ld);\n' if npd else '')
-    return '''extern "C" {{
+                 '(K, T, P, C, pres_mod, n, ld);\n' if npd else '')
+    return '''// Device address of the constant table, looked up once per device.
+static int table(const char *fn, const double **out)
+{{
+    static std::mutex lock;
+    static const double *addr[64] = {{nullptr}};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return crs::fail(CRS_ERR_HIP, "%s: no current HIP device", fn);
+    std::lock_guard<std::mutex> g(lock);
+    if (addr[dev] == nullptr) {{
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(crs_K));
+        if (e != hipSuccess)
+            return crs::fail(CRS_ERR_HIP, "%s: constant table: %s", fn,
+                             hipGetErrorString(e));
+        addr[dev] = (const double *)p;
+    }}
+    *out = addr[dev];
+    return CRS_OK;
+}}
+
+extern "C" {{
 
 int crs_get_mech_info(crs_mech_info *out)
 {{
@@ -682,7 +974,9 @@ int crs_eval_conc(const double *T, const double *P, const double *Y,
     CRS_TRY(crs::check_ptr(fn, conc, "conc", n));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
-    k_eval_conc<<<crs::grid_for(n), crs::kBlock, 0, s>>>(T, P, Y, y_N, mw_avg,
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    k_eval_conc<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, P, Y, y_N, mw_avg,
                                                          rho, conc, n, ld);
     return crs::launched(fn);
 }}
@@ -700,7 +994,9 @@ int crs_eval_rxn_rates(const double *T, const double *P, const double *C,
     CRS_TRY(crs::check_ptr(fn, rev, "rev", NREV > 0 ? n : 0));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
-    k_eval_rxn_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(T, C, fwd, rev,
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    k_eval_rxn_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, C, fwd, rev,
                                                               n, ld);
     return crs::launched(fn);
 }}
@@ -718,6 +1014,8 @@ int crs_get_rxn_pres_mod(const double *T, const double *P, const double *C,
     CRS_TRY(crs::check_ptr(fn, pres_mod, "pres_mod", n));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
 {pm_launch}    return crs::launched(fn);
 }}
 
@@ -734,8 +1032,10 @@ int crs_eval_spec_rates(const double *fwd, const double *rev,
     CRS_TRY(crs::check_ptr(fn, dy_N, "dy_N", n));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
     k_eval_spec_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(
-        fwd, rev, pres_mod, sp_rates, dy_N, n, ld);
+        K, fwd, rev, pres_mod, sp_rates, dy_N, n, ld);
     return crs::launched(fn);
 }}
 
@@ -760,7 +1060,10 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
     CRS_TRY(crs::check_ptr(fn, wdot, "wdot", n));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
-    k_wdot_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(T, P, Y, wdot, n, ld);
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    k_wdot_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, P, Y, wdot,
+                                                        nullptr, n, ld);
     return crs::launched(fn);
 }}
 
@@ -781,8 +1084,10 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
     CRS_TRY(crs::check_ptr(fn, idx, "idx", m));
     if (m == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
-    k_wdot_f64_subset<<<crs::grid_for(m), crs::kBlock, 0, s>>>(T, P, Y, wdot,
-                                                               idx, m, ld);
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    k_wdot_f64<<<crs::grid_for(m), crs::kBlock, 0, s>>>(K, T, P, Y, wdot, idx,
+                                                        m, ld);
     return crs::launched(fn);
 }}
 
@@ -790,12 +1095,13 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
 '''.format(name=ir.name, pm_launch=pm_launch)
 
 
-def write_hip(out_dir, specs, reacs, name='mech', spec_order=None):
+def write_hip(out_dir, specs, reacs, name='mech', spec_order=None, opts=None,
+              filename=None):
     """Emit ``crs_<name>.hip`` into ``out_dir``; return its path."""
     ir = build_ir(specs, reacs, name=name)
     os.makedirs(out_dir, exist_ok=True)
-    path = os.path.join(out_dir, 'crs_{}.hip'.format(name))
-    text = emit_hip_source(ir)
+    path = os.path.join(out_dir, filename or 'crs_{}.hip'.format(name))
+    text = emit_hip_source(ir, opts)
     old = None
     if os.path.exists(path):
         with open(path) as fh:

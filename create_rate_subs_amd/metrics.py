@@ -22,6 +22,10 @@ import numpy as np
 __all__ = ['nu_matrix', 'gross_scale', 'rel_err', 'summarize']
 
 
+def _is_torch(x):
+    return type(x).__module__.split('.')[0] == 'torch'
+
+
 def nu_matrix(specs, reacs):
     """Net stoichiometric matrix [NSP][NR] (products minus reactants)."""
     nu = np.zeros((len(specs), len(reacs)))
@@ -34,7 +38,7 @@ def nu_matrix(specs, reacs):
 
 
 def _gross_terms(reacs, fwd, rev, pm, xp):
-    g = xp.abs(fwd).clone() if hasattr(fwd, 'clone') else np.abs(fwd).copy()
+    g = xp.abs(fwd).clone() if _is_torch(fwd) else np.abs(fwd).copy()
     ri = pi = 0
     for j, r in enumerate(reacs):
         if r.rev:
@@ -49,7 +53,7 @@ def _gross_terms(reacs, fwd, rev, pm, xp):
 def gross_scale(specs, reacs, fwd, rev, pm):
     """S_k for every state; works on numpy arrays or torch tensors."""
     nu = np.abs(nu_matrix(specs, reacs))
-    if hasattr(fwd, 'device'):
+    if _is_torch(fwd):
         import torch
         g = _gross_terms(reacs, fwd, rev, pm, torch)
         return torch.as_tensor(nu, dtype=g.dtype, device=g.device) @ g
@@ -59,7 +63,7 @@ def gross_scale(specs, reacs, fwd, rev, pm):
 
 def rel_err(w_hat, w, S, tau):
     """Element-wise conditioned relative error (numpy or torch)."""
-    if hasattr(w, 'device'):
+    if _is_torch(w):
         import torch
         d = (w_hat - w).abs()
         den = torch.maximum(w.abs(), tau * S)
@@ -75,7 +79,7 @@ def rel_err(w_hat, w, S, tau):
 
 def summarize(err):
     """max / median / 99.9th percentile of an error array."""
-    if hasattr(err, 'device'):
+    if _is_torch(err):
         e = err.flatten().double()
         n = e.numel()
         k = max(1, int(n * 0.999))

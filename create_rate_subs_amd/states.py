@@ -63,7 +63,7 @@ def species_smh(specs, T):
 
 
 def equilibrium_states(specs, n, seed, perturb=1e-4, t_range=(1800.0, 2500.0),
-                       bath_mole_fraction=0.6, spread=3.0):
+                       bath_mole_fraction=0.6, spread=1.0):
     """``n`` element-potential equilibrium states, perturbed by ``perturb``.
 
     ``specs`` are the parsed species (bath gas last); their element

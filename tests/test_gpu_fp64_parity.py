@@ -19,9 +19,12 @@ from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 MECHS = ['mech13', 'mech53', 'mech111']
-# tau of the conditioned error: below tau * S_k, errors count relative to
-# tau * S_k.  1e-3 * S_k * 1e-12 = 1e-15 S_k, a few ulps of the terms.
-TAU = 1e-3
+# tau of the conditioned error: where |wdot_k| < tau * S_k, errors count
+# relative to tau * S_k.  Two correct FP64 evaluations of exp(x) with
+# |x| up to ~90 differ by up to |x| ulp in the result whenever x itself is
+# rounded differently (FMA contraction, a one-ulp log(T)), i.e. ~1e-14
+# relative per term; tau = 1e-2 makes that floor 1e-14 S_k = 1e-12 tau S_k.
+TAU = 1e-2
 
 
 def _states(name, n, seed):
@@ -79,6 +82,8 @@ def test_wdot_fp64_matches_oracle(name, n):
     S = metrics.gross_scale(specs, reacs, ref['fwd'], ref['rev'], ref['pm'])
     err = metrics.rel_err(W, ref['wdot'], S, TAU)
     s = metrics.summarize(err)
+    s['tau'] = TAU
+    s['pointwise_max'] = float(metrics.rel_err(W, ref['wdot'], S, 0.0).max())
     print(name, n, s)
     assert s['max'] <= 1e-12, s
 

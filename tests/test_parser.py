@@ -1,0 +1,204 @@
+"""Chemkin / NASA-7 parsing: hand-computed unit conversions and bookkeeping.
+
+Every expected number is worked out by hand from the reference's rules
+(mech_interpret.py:39-46 activation-energy factors, :441-455 / :519-520 /
+:538-539 pre-exponential orders, :673-693 REV split, :775-858 thermo
+columns, create_rate_subs.py:473-526 bath-gas reordering).
+"""
+
+import math
+import os
+
+import pytest
+
+from create_rate_subs_amd import chem, mech_interpret as mi
+from create_rate_subs_amd.create_rate_subs import load_mechanism
+from create_rate_subs_amd.utils import get_species_mappings
+
+THERMO = """THERMO
+   300.000  1000.000  5000.000
+H                 SYN   H   1               G   300.000  5000.000 1000.00      1
+ 2.50000000E+00 0.00000000E+00 0.00000000E+00 0.00000000E+00 0.00000000E+00    2
+ 2.54736599E+04-4.46682853E-01 2.50000000E+00 0.00000000E+00 0.00000000E+00    3
+ 0.00000000E+00 0.00000000E+00 2.54736599E+04-4.46682850E-01                   4
+O2                SYN   O   2               G   300.000  5000.000 1000.00      1
+ 3.28253784E+00 1.48308754E-03-7.57966669E-07 2.09470555E-10-2.16717794E-14    2
+-1.08845772E+03 5.45323129E+00 3.78245636E+00-2.99673416E-03 9.84730201E-06    3
+-9.68129509E-09 3.24372837E-12-1.06394356E+03 3.65767573E+00                   4
+HO2               SYN   H   1O   2          G   300.000  5000.000 1000.00      1
+ 4.01721090E+00 2.23982013E-03-6.33658150E-07 1.14246370E-10-1.07908535E-14    2
+ 1.11856713E+02 3.78510215E+00 4.30179801E+00-4.74912051E-03 2.11582891E-05    3
+-2.42763894E-08 9.29225124E-12 2.94808040E+02 3.71666245E+00                   4
+OH                SYN   O   1H   1          G   300.000  5000.000  1357.00     1
+ 3.09288767E+00 5.48429716E-04 1.26505228E-07-8.79461556E-11 1.17412376E-14    2
+ 3.85865700E+03 4.47669610E+00 3.99201543E+00-2.40131752E-03 4.61793841E-06    3
+-3.88113333E-09 1.36411470E-12 3.61508056E+03-1.03925458E-01                   4
+N2                SYN   N   2               G   300.000  5000.000 1000.00      1
+ 2.92664000E+00 1.48797680E-03-5.68476000E-07 1.00970380E-10-6.75335100E-15    2
+-9.22797700E+02 5.98052800E+00 3.29867700E+00 1.40824040E-03-3.96322200E-06    3
+ 5.64151500E-09-2.44485400E-12-1.02089990E+03 3.95037200E+00                   4
+CH2(S)            SYN   C   1H   2          G   300.000  5000.000 1000.00      1
+ 2.29203842E+00 4.65588637E-03-2.01191947E-06 4.17906000E-10-3.39716365E-14    2
+ 5.09259997E+04 8.62650169E+00 4.19860411E+00-2.36661419E-03 8.23296220E-06    3
+-6.68815981E-09 1.94314737E-12 5.04968163E+04-7.69118967E-01                   4
+END
+"""
+
+MECH = """ELEMENTS
+H O N C
+END
+SPECIES
+H O2 N2 HO2 OH CH2(S)
+END
+REACTIONS
+H + O2 <=> OH + OH   1.0E+13  0.0  1000.0
+H + O2 + M <=> HO2 + M   2.0E+18  -0.8  0.0
+  O2/0.4/ N2/1.5/
+H + O2 (+M) <=> HO2 (+M)   4.0E+12  0.5  0.0
+  LOW / 6.0E+19 -1.4 0.0 /
+  TROE / 0.5 100.0 2000.0 /
+H + O2 (+N2) <=> HO2 (+N2)   4.0E+12  0.5  0.0
+  HIGH / 6.0E+10 1.0 500.0 /
+  SRI / 1.1 300.0 900.0 /
+2OH <=> H + HO2   5.0E+10  1.5  2.0
+  REV / 3.0E+11 0.5 4.0 /
+  DUP
+CH2(S) + M => CH2(S) + M  1.0E+05  0.0  0.0
+1.5O2 + H => HO2 + O2   1.0E+10  0.0  0.0
+END
+"""
+
+
+@pytest.fixture()
+def files(tmp_path):
+    m = tmp_path / 'mech.dat'
+    t = tmp_path / 'therm.dat'
+    m.write_text(MECH)
+    t.write_text(THERMO)
+    return str(m), str(t)
+
+
+def test_species_and_thermo(files):
+    elems, specs, reacs = mi.read_mech(*files)
+    assert elems == ['H', 'O', 'N', 'C']
+    assert [s.name for s in specs] == ['H', 'O2', 'N2', 'HO2', 'OH', 'CH2(S)']
+    ho2 = specs[3]
+    assert ho2.elem == [['H', 1], ['O', 2]]
+    assert ho2.mw == 0.0 + 1 * 1.00794 + 2 * 15.9994
+    assert ho2.hi[0] == 4.01721090 and ho2.lo[6] == 3.71666245
+    assert ho2.hi[5] == 1.11856713e2 and ho2.lo[0] == 4.30179801
+    assert specs[4].Trange == [300.0, 1357.0, 5000.0]      # own T_mid
+    assert specs[0].Trange == [300.0, 1000.0, 5000.0]      # common line
+
+
+def test_units_and_forms(files):
+    _e, specs, reacs = mi.read_mech(*files)
+    f_cal = 4.184 / 8.3144621
+    r0 = reacs[0]
+    # bimolecular elementary: cm^3/mol/s -> m^3/kmol/s, divide by 1000
+    assert r0.A == 1.0e13 / 1000.0 and r0.rev and not r0.thd_body
+    assert r0.E == 1000.0 * f_cal
+    assert r0.prod == ['OH'] and r0.prod_nu == [2]
+    # third body: M not counted, divide by 1000^2
+    r1 = reacs[1]
+    assert r1.thd_body and not r1.pdep
+    assert r1.A == 2.0e18 / 1000.0 ** 2 and r1.b == -0.8 and r1.E == 0.0
+    assert r1.thd_body_eff == [['O2', 0.4], ['N2', 1.5]]
+    # falloff (+M): main (k_inf) / 1000^(order-1), LOW / 1000^order
+    r2 = reacs[2]
+    assert r2.pdep and not r2.thd_body and r2.pdep_sp == ''
+    assert r2.A == 4.0e12 / 1000.0
+    assert r2.low == [6.0e19 / 1000.0 ** 2, -1.4, 0.0]
+    assert r2.troe and r2.troe_par == [0.5, 100.0, 2000.0]
+    # named collider, HIGH / 1000^(order-2), SRI
+    r3 = reacs[3]
+    assert r3.pdep and r3.pdep_sp == 'N2'
+    assert r3.high == [6.0e10 / 1000.0 ** 0, 1.0, 500.0 * f_cal]
+    assert r3.sri and r3.sri_par == [1.1, 300.0, 900.0]
+    # REV split into two irreversible reactions, reverse A by product order
+    r4, r5 = reacs[4], reacs[5]
+    assert not r4.rev and not r5.rev and r4.dup and r5.dup
+    assert r4.reac == ['OH'] and r4.reac_nu == [2]
+    assert r5.reac == ['H', 'HO2'] and r5.prod == ['OH']
+    assert r5.A == 3.0e11 / 1000.0 and r5.E == 4.0 * f_cal and r5.b == 0.5
+    # species name with parentheses, bare M on both sides
+    r6 = reacs[6]
+    assert r6.thd_body and r6.reac == ['CH2(S)'] and not r6.rev
+    assert r6.A == 1.0e5 / 1000.0
+    # fractional coefficient
+    r7 = reacs[7]
+    assert r7.reac == ['O2', 'H'] and r7.reac_nu == [1.5, 1]
+    assert isinstance(r7.reac_nu[0], float)
+    assert r7.A == 1.0e10 / 1000.0 ** 1.5
+
+
+def test_kcal_units(tmp_path):
+    m = tmp_path / 'm.dat'
+    m.write_text(MECH.replace('REACTIONS', 'REACTIONS KCAL/MOLE'))
+    t = tmp_path / 't.dat'
+    t.write_text(THERMO)
+    _e, _s, reacs = mi.read_mech(str(m), str(t))
+    assert reacs[0].E == 1000.0 * 4184. / 8.3144621
+
+
+def test_unsupported_units(tmp_path):
+    m = tmp_path / 'm.dat'
+    m.write_text(MECH.replace('REACTIONS', 'REACTIONS FURLONGS'))
+    t = tmp_path / 't.dat'
+    t.write_text(THERMO)
+    with pytest.raises(mi.MechanismError):
+        mi.read_mech(str(m), str(t))
+    m.write_text(MECH.replace('REACTIONS', 'REACTIONS MOLECULES'))
+    with pytest.raises(NotImplementedError):
+        mi.read_mech(str(m), str(t))
+
+
+def test_missing_thermo(tmp_path):
+    m = tmp_path / 'm.dat'
+    m.write_text(MECH)
+    t = tmp_path / 't.dat'
+    t.write_text(THERMO.replace('HO2 ', 'XO2 '))
+    with pytest.raises(mi.MechanismError, match='HO2'):
+        mi.read_mech(str(m), str(t))
+
+
+def test_thermo_without_end_terminates(tmp_path):
+    m = tmp_path / 'm.dat'
+    m.write_text(MECH)
+    t = tmp_path / 't.dat'
+    t.write_text(THERMO.replace('END\n', ''))
+    _e, specs, _r = mi.read_mech(str(m), str(t))
+    assert all(s.mw for s in specs)
+
+
+def test_bath_gas_reordering(files):
+    _e, specs, reacs, fwd = load_mechanism(*files)
+    # N2 (index 2, MW exactly 2 * 14.00674) moves to the end
+    assert [s.name for s in specs][-1] == 'N2'
+    assert fwd == [0, 1, 3, 4, 5, 2]
+    assert reacs[0].reac == [0, 1] and reacs[0].prod == [3]
+    assert reacs[1].thd_body_eff == [(1, 0.4), (5, 1.5)]
+    assert reacs[3].pdep_sp == 5
+    _e, specs, _r, fwd = load_mechanism(*files, last_spec='oh')
+    assert specs[-1].name == 'OH'
+    _e, specs, _r, fwd = load_mechanism(*files, last_spec='XX')
+    assert specs[-1].name == 'N2'
+
+
+def test_species_mappings():
+    fwd, back = get_species_mappings(5, 1)
+    assert fwd == [0, 2, 3, 4, 1]
+    assert back == [0, 4, 1, 2, 3]
+
+
+def test_calc_spec_smh_closed_form(files):
+    _e, specs, _r = mi.read_mech(*files)
+    sp = specs[1]
+    for T in (500.0, 1000.0, 1000.0001, 2200.0):
+        a = sp.lo if T <= 1000.0 else sp.hi
+        s = (a[0] * math.log(T) + a[1] * T + a[2] * T * T / 2 +
+             a[3] * T ** 3 / 3 + a[4] * T ** 4 / 4 + a[6])
+        h = (a[0] + a[1] * T / 2 + a[2] * T * T / 3 + a[3] * T ** 3 / 4 +
+             a[4] * T ** 4 / 5 + a[5] / T)
+        got = chem.calc_spec_smh(T, [sp])[0]
+        assert abs(got - (s - h)) <= 1e-12 * max(1.0, abs(s - h))
