@@ -1,16 +1,14 @@
 #!/usr/bin/env python3
 """Throughput of the species-rate (wdot) path on one MI355X.
 
-One step = one pass of the fused wdot kernel over the whole state batch,
-inputs resident in HBM.  Default workload: the 53-species / 325-reaction
-synthetic mechanism over 2^20 seeded FP64 states, FP64 path (the drop-in).
+One step = one launch of the fused wdot kernel over the whole state batch,
+inputs resident in HBM.  Headline: the 53-species / 325-reaction synthetic
+mechanism over 2^20 seeded FP64 states on the FP64 path (the drop-in).
 
-Prints one JSON line: states/s, the dominant kernel's roofline position
-(algorithmic flops / its event-timed duration against MI355X's FP64 peak),
-and the C oracle's throughput on the host cores over a bounded sample of
-the same workload.  The mixed-precision and surrogate paths, when built,
-are timed over the same states and reported with their error against the
-FP64 path.
+The same JSON line also carries the mixed-precision path on 2^22 states
+(plus a 2^18 near-equilibrium slice) with its speed-up over the FP64 kernel
+on the same states and its conditioned relative error against FP64, and the
+C oracle's throughput on the host cores over a bounded sample.
 """
 
 import argparse
@@ -27,6 +25,7 @@ import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+TAU = 5e-2
 
 
 def parse():
@@ -41,6 +40,7 @@ def parse():
                     help='states in the bounded CPU-baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-mixed', action='store_true')
     return ap.parse_args()
 
 
@@ -61,10 +61,56 @@ def cpu_baseline(name, T, P, Y, sample, threads):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     return dict(value=m / t, unit='states/s', cores=threads, kind='port',
-                sample='{} of the workload\'s states, {} threads, median of 3 '
-                       'after 1 warm-up ({:.2f} s each); host {}'.format(
-                           m, threads, t, platform.processor() or
-                           platform.machine()))
+                sample='{} of the workload\'s states, {} OpenMP threads, '
+                       'median of 3 after 1 warm-up ({:.3f} s each); host {}'
+                       .format(m, threads, t, platform.processor() or
+                               platform.machine()))
+
+
+def timed(torch, fn, steps, warmup):
+    """Run fn warmup times, then time steps launches with events and wall."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record()
+        fn()
+        ends[k].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    return wall * 1e3 / steps, kern
+
+
+def gross_scale_dev(torch, lib, specs, reacs, T, P, Y, chunk=1 << 19):
+    """S_k of every state on the GPU, from the FP64 stage kernels, in chunks."""
+    from create_rate_subs_amd import metrics
+    n = T.numel()
+    S = torch.empty_like(Y)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        Tc, Pc = T[a:b].contiguous(), P[a:b].contiguous()
+        C = lib.eval_conc(Tc, Pc, Y[:, a:b].contiguous())['C']
+        fwd, rev = lib.eval_rxn_rates(Tc, Pc, C)
+        pm = lib.get_rxn_pres_mod(Tc, Pc, C)
+        S[:, a:b] = metrics.gross_scale(specs, reacs, fwd, rev, pm)
+        del C, fwd, rev, pm
+    return S
+
+
+def err_summary(torch, W, ref, S):
+    from create_rate_subs_amd import metrics
+    e = metrics.rel_err(W, ref, S, TAU)
+    s = metrics.summarize(e)
+    ok = (ref.abs() >= TAU * S) & (S > 0)
+    pw = ((W - ref).abs() / ref.abs())[ok]
+    s['tau'] = TAU
+    s['well_conditioned_pointwise_max'] = pw.max().item() if pw.numel() else 0.0
+    s['well_conditioned_fraction'] = ok.double().mean().item()
+    return s
 
 
 def main():
@@ -92,20 +138,9 @@ def main():
     Wd = torch.empty_like(Yd)
     torch.cuda.synchronize()
 
-    for _ in range(args.warmup):
-        lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd)
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        starts[k].record()
-        lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd)
-        ends[k].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step, kern_ms = timed(
+        torch, lambda: lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd),
+        args.steps, args.warmup)
     value = n / (ms_per_step * 1e-3)
 
     fl = cost.flops_for(specs, reacs)['total']
@@ -119,8 +154,8 @@ def main():
                     hbm_GBs=by * n / (kern_ms * 1e-3) / 1e9,
                     hbm_frac=by * n / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                     note='FP64 VALU-bound; peak is MI355X FP64 (vector and '
-                         'matrix are both 78.6 TFLOP/s)')
-
+                         'matrix both 78.6 TFLOP/s); flops counted as in the '
+                         'reference\'s emitted expressions, exp/log = 1')
     line = dict(metric='wdot states/s (53sp/325rxn): FP64 vs mixed vs '
                        'surrogate @ stated rel err',
                 value=value, unit='states/s', n_gpus=1, steps=args.steps,
@@ -137,6 +172,40 @@ def main():
             'OMP_NUM_THREADS', min(16, os.cpu_count() or 1)))
         line['cpu_baseline'] = cpu_baseline(wl.mech, T, P, Y,
                                             args.cpu_sample, threads)
+    del Td, Pd, Yd, Wd
+
+    if not args.no_mixed and (lib.modes & 2):
+        w2 = workloads.WORKLOADS['w2_53']
+        T2, P2, Y2 = states.random_states(nsp, w2.n_states, w2.state_seed)
+        T2d = torch.from_numpy(T2).cuda()
+        P2d = torch.from_numpy(P2).cuda()
+        Y2d = torch.from_numpy(Y2).cuda()
+        W64 = torch.empty_like(Y2d)
+        Wmx = torch.empty_like(Y2d)
+        steps = max(3, args.steps // 2)
+        f64_ms, f64_k = timed(torch, lambda: lib.eval_wdot(
+            T2d, P2d, Y2d, 'fp64', out=W64), steps, 1)
+        mx_ms, mx_k = timed(torch, lambda: lib.eval_wdot(
+            T2d, P2d, Y2d, 'mixed', out=Wmx), steps, 1)
+        S = gross_scale_dev(torch, lib, specs, reacs, T2d, P2d, Y2d)
+        mixed = dict(workload='w2_53: {} states'.format(w2.n_states),
+                     states_per_s=w2.n_states / (mx_ms * 1e-3),
+                     fp64_states_per_s=w2.n_states / (f64_ms * 1e-3),
+                     kernel_ms=mx_k, fp64_kernel_ms=f64_k,
+                     speedup_vs_fp64=f64_k / mx_k,
+                     err_vs_fp64=err_summary(torch, Wmx, W64, S))
+        del T2d, P2d, Y2d, W64, Wmx, S
+        Te, Pe, Ye = states.equilibrium_states(specs, w2.near_eq_states,
+                                               w2.state_seed + 100)
+        Ted = torch.from_numpy(Te).cuda()
+        Ped = torch.from_numpy(Pe).cuda()
+        Yed = torch.from_numpy(Ye).cuda()
+        We64 = lib.eval_wdot(Ted, Ped, Yed, 'fp64')
+        Wemx = lib.eval_wdot(Ted, Ped, Yed, 'mixed')
+        Se = gross_scale_dev(torch, lib, specs, reacs, Ted, Ped, Yed)
+        mixed['near_eq'] = dict(states=w2.near_eq_states,
+                                err_vs_fp64=err_summary(torch, Wemx, We64, Se))
+        line['mixed'] = mixed
     print(json.dumps(line), flush=True)
 
 

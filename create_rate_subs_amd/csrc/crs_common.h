@@ -70,6 +70,59 @@ inline dim3 grid_for(int64_t n)
     return dim3((unsigned)((n + kBlock - 1) / kBlock));
 }
 
+// 1/x to within an ulp: hardware reciprocal estimate refined by two
+// Newton-Raphson steps (each doubles the correct bits).  For positive
+// normal x away from the overflow boundary.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision building blocks.  The argument of an exponential is formed
+// and range-reduced in FP64 (x log2 e = n + f, |f| <= 1/2, exact since
+// n = rint(x log2 e)); only 2^f is taken on the FP32 hardware exponential
+// (v_exp_f32), then rescaled by 2^n.  The relative error is that of v_exp_f32
+// on [-1/2, 1/2] plus the FP32 rounding of f, about 1e-7, independent of |x|.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double exp_mixed(double x)
+{
+    const double t = x * 1.4426950408889634;   // log2(e)
+    const double n = __builtin_rint(t);
+    const float f = (float)(t - n);
+    return __builtin_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+// Same reduction, FP32 result (for arguments whose result fits FP32).
+__device__ __forceinline__ float expf_mixed(double x)
+{
+    const double t = x * 1.4426950408889634;
+    const double n = __builtin_rint(t);
+    const float f = (float)(t - n);
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+// log10 of a positive normal FP32 value on the hardware log2.
+__device__ __forceinline__ float log10f_hw(float x)
+{
+    return __builtin_amdgcn_logf(x) * 0.30102999566398120f;
+}
+
+// 10^x for |x| of order one on the hardware exp2.
+__device__ __forceinline__ float exp10f_hw(float x)
+{
+    return __builtin_amdgcn_exp2f(x * 3.3219280948873623f);
+}
+
+__device__ __forceinline__ float rcpf_hw(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+
 }  // namespace crs
 
 #define CRS_TRY(expr)                 \

@@ -92,7 +92,7 @@ def equilibrium_states(specs, n, seed, perturb=1e-4, t_range=(1800.0, 2500.0),
     # ln sum exp(base + atoms*s) is increasing in s, so bisect
     lo = np.full(n, -400.0)
     hi = np.full(n, 400.0)
-    for _ in range(200):
+    for _ in range(64):
         mid = 0.5 * (lo + hi)
         v = base + atoms[:, None] * mid
         m = v.max(axis=0)

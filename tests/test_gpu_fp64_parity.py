@@ -20,11 +20,12 @@ pytestmark = pytest.mark.gpu
 
 MECHS = ['mech13', 'mech53', 'mech111']
 # tau of the conditioned error: where |wdot_k| < tau * S_k, errors count
-# relative to tau * S_k.  Two correct FP64 evaluations of exp(x) with
-# |x| up to ~90 differ by up to |x| ulp in the result whenever x itself is
-# rounded differently (FMA contraction, a one-ulp log(T)), i.e. ~1e-14
-# relative per term; tau = 1e-2 makes that floor 1e-14 S_k = 1e-12 tau S_k.
-TAU = 1e-2
+# relative to tau * S_k.  Every rate constant is exp(x) with |x| up to
+# ~100 (ln A, E/T and the NASA polynomial of ln Kc), and two correct FP64
+# evaluations of x round differently by up to |x| * 2^-52 ~ 2e-14, which
+# is then the relative difference of the term; near cancellation that
+# floor is 2e-14 S_k.  tau = 5e-2 puts the test's floor at 5e-14 S_k.
+TAU = 5e-2
 
 
 def _states(name, n, seed):
