@@ -41,6 +41,9 @@ def parse():
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
+    ap.add_argument('--no-surrogate', action='store_true')
+    ap.add_argument('--sur-epochs', type=int, default=30)
+    ap.add_argument('--sur-target', type=float, default=1e-2)
     return ap.parse_args()
 
 
@@ -111,6 +114,71 @@ def err_summary(torch, W, ref, S):
     s['well_conditioned_pointwise_max'] = pw.max().item() if pw.numel() else 0.0
     s['well_conditioned_fraction'] = ok.double().mean().item()
     return s
+
+
+def surrogate_section(args, torch, lib, specs, reacs):
+    """Train on 2^22 exact-labelled states, calibrate the gate on held-out
+    states, then time gated inference with exact fallback on fresh states."""
+    from create_rate_subs_amd import metrics, states, workloads
+    from create_rate_subs_amd.surrogate import Surrogate
+    w4 = workloads.WORKLOADS['w4_53']
+    nsp = len(specs)
+    n = w4.n_states
+
+    def labelled(count, seed):
+        T, P, Y = states.random_states(nsp, count, seed)
+        Td, Pd, Yd = (torch.from_numpy(a).cuda() for a in (T, P, Y))
+        return Td, Pd, Yd, lib.eval_wdot(Td, Pd, Yd, 'fp64')
+
+    out = dict(workload='w4_53: train {} states, test {} states'.format(n, n))
+    Tt, Pt, Yt, Wt = labelled(n, w4.state_seed)
+    net = Surrogate(lib, specs, reacs, seed=w4.state_seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train(Tt, Pt, Yt, Wt, epochs=args.sur_epochs, batch=16384)
+    torch.cuda.synchronize()
+    ttrain = time.perf_counter() - t0
+    out['train'] = dict(states=n, epochs=args.sur_epochs, seconds=ttrain,
+                        state_epochs_per_s=n * args.sur_epochs / ttrain,
+                        mse_first=hist[0], mse_last=hist[-1])
+    del Tt, Pt, Yt, Wt
+    # gate calibration on held-out states
+    Tv, Pv, Yv, Wv = labelled(1 << 18, w4.state_seed + 100)
+    Sv = gross_scale_dev(torch, lib, specs, reacs, Tv, Pv, Yv)
+    rmax, admitted, curve = net.calibrate_gate(Tv, Pv, Yv, Wv, Sv,
+                                               args.sur_target, TAU)
+    out['gate'] = dict(target=args.sur_target, tau=TAU, rmax=rmax,
+                       validation_admitted=admitted,
+                       tradeoff_on_validation=curve)
+    del Tv, Pv, Yv, Wv, Sv
+    # fresh test states
+    Te, Pe, Ye, We = labelled(n, w4.state_seed + 200)
+    Wout = torch.empty_like(We)
+    steps = 3
+    f64_ms, _k = timed(torch, lambda: lib.eval_wdot(Te, Pe, Ye, 'fp64',
+                                                     out=Wout), steps, 1)
+    pred_ms, _k = timed(torch, lambda: net.predict(Te, Pe, Ye, Wout),
+                        steps, 1)
+    fb = []
+
+    def gated():
+        fb.append(net.evaluate(Te, Pe, Ye, Wout))
+    gated_ms, _k = timed(torch, gated, steps, 1)
+    S = gross_scale_dev(torch, lib, specs, reacs, Te, Pe, Ye)
+    e = metrics.rel_err(Wout, We, S, TAU)
+    per_state = e.max(dim=0).values
+    out['test'] = dict(
+        states=n, gated_states_per_s=n / (gated_ms * 1e-3),
+        surrogate_only_states_per_s=n / (pred_ms * 1e-3),
+        fp64_states_per_s=n / (f64_ms * 1e-3),
+        speedup_vs_fp64=f64_ms / gated_ms,
+        fallback_rate=fb[-1] / n,
+        post_fallback_err=dict(max=e.max().item(),
+                               state_median=per_state.median().item(),
+                               state_p99=per_state[::16].quantile(0.99).item(),
+                               tau=TAU),
+        audit_1pct=net.audit(Te, Pe, Ye, Wout, frac=0.01))
+    return out
 
 
 def main():
@@ -206,6 +274,8 @@ def main():
         mixed['near_eq'] = dict(states=w2.near_eq_states,
                                 err_vs_fp64=err_summary(torch, Wemx, We64, Se))
         line['mixed'] = mixed
+    if not args.no_surrogate:
+        line['surrogate'] = surrogate_section(args, torch, lib, specs, reacs)
     print(json.dumps(line), flush=True)
 
 

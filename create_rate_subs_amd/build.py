@@ -13,7 +13,7 @@ from . import hip_emitter, workloads
 from .create_rate_subs import load_mechanism
 
 __all__ = ['HIPCC_FLAGS', 'compile_library', 'build_mechanism', 'lib_path',
-           'INCLUDE_DIRS']
+           'INCLUDE_DIRS', 'build_mlp']
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -44,6 +44,28 @@ def compile_library(src, out_dir=None, extra_flags=(), verbose=False):
     if res.returncode != 0:
         raise RuntimeError('hipcc failed for {}:\n{}'.format(
             src, res.stderr[-4000:]))
+    os.replace(lib + '.tmp', lib)
+    if verbose:
+        print('built {} in {:.1f} s'.format(lib, time.time() - t0))
+    return lib
+
+
+MLP_SRC = os.path.join(HERE, 'csrc', 'mlp.hip')
+
+
+def build_mlp(verbose=False):
+    """Compile the mechanism-independent surrogate library."""
+    out_dir = workloads.BUILD_ROOT
+    os.makedirs(out_dir, exist_ok=True)
+    lib = os.path.join(out_dir, 'libcrs_mlp.so')
+    cmd = [hipcc()] + HIPCC_FLAGS
+    for d in INCLUDE_DIRS:
+        cmd += ['-I', d]
+    cmd += ['-o', lib + '.tmp', MLP_SRC]
+    t0 = time.time()
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('hipcc failed for mlp.hip:\n' + res.stderr[-4000:])
     os.replace(lib + '.tmp', lib)
     if verbose:
         print('built {} in {:.1f} s'.format(lib, time.time() - t0))

@@ -1,0 +1,103 @@
+/*
+ * C ABI of the wdot surrogate library (libcrs_mlp.so).
+ *
+ * Mechanism-independent kernels for a small MLP that maps per-state
+ * features (ln T, ln P, ln Y_k) to arcsinh(wdot_k / s_k): BF16 MFMA matrix
+ * products with FP32 accumulation, GELU, MSE gradient, Adam with FP32
+ * master weights, feature/target/decode transforms and the fallback gate.
+ * No counterpart in the reference (its rates are always evaluated exactly);
+ * callers are the surrogate training/inference driver
+ * (create_rate_subs_amd/surrogate.py).
+ *
+ * Conventions: activations are state-major [rows][ld] BF16 (raw 16-bit
+ * bf16 in `void *`), weights [out][in] BF16 with in/out padded to multiples
+ * of 64 (padding zero); the reduction length of every product is a
+ * multiple of 32.  All pointers are device pointers; work is enqueued on
+ * `stream` (a hipStream_t, NULL = default).  Return 0 or a CRS_ERR_* code;
+ * crs_mlp_last_error() gives the message.
+ */
+#ifndef CRS_MLP_H
+#define CRS_MLP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct crs_mlp_gate_params {
+    double t_lo, t_hi; /* training box in T [K] */
+    double p_lo, p_hi; /* training box in P [Pa] */
+    double rmax;       /* max element / mass balance residual, relative */
+} crs_mlp_gate_params;
+
+const char *crs_mlp_last_error(void);
+
+/* Z = X W^T + b; act != 0: A_out = gelu(Z) (bf16), Z_out (fp32, optional);
+ * act == 0: Y_out = Z (fp32) for columns < n_valid.  X [M][ldx], W [N][ldw],
+ * outputs [M][ldo].  N % 64 == 0, K % 32 == 0. */
+int crs_mlp_linear_fwd(const void *X, int64_t ldx, const void *W,
+                       int64_t ldw, const float *bias, int64_t M, int N,
+                       int K, int act, void *A_out, float *Z_out, float *Y_out,
+                       int64_t ldo, int n_valid, void *stream);
+
+/* dZprev = (dZ W) * gelu'(Zprev): dZ [M][ldz] (N wide), W [N][ldw] (K wide),
+ * Zprev and dZprev [M][ldo] (K wide). */
+int crs_mlp_linear_bwd_data(const void *dZ, int64_t ldz, const void *W,
+                            int64_t ldw, int64_t M, int N, int K,
+                            const float *Zprev, void *dZprev, int64_t ldo,
+                            void *stream);
+
+/* dW[N][K] += dZ^T X and db[N] += colsum(dZ) (FP32 atomics; zero them
+ * first). */
+int crs_mlp_linear_bwd_weight(const void *dZ, int64_t ldz, const void *X,
+                              int64_t ldx, int64_t M, int N, int K, float *dW,
+                              float *db, void *stream);
+
+/* X[t][0..kp) = normalised [ln T, ln P, ln max(Y_k, 1e-30) for k < nsp]
+ * of state idx[t] (or t when idx is NULL), the last species taken as
+ * 1 - sum(others). */
+int crs_mlp_features(const double *T, const double *P, const double *Y,
+                     const int32_t *idx, int64_t n, int64_t ld, int nsp,
+                     const float *mu, const float *sd, void *X, int kp,
+                     void *stream);
+
+/* Yt[t][k] = (asinh(W[k][t] / scale[k]) - mu[k]) / sd[k]. */
+int crs_mlp_targets(const double *W, int64_t n, int64_t ld, int nsp,
+                    const double *scale, const float *mu, const float *sd,
+                    float *Yt, int ldt, void *stream);
+
+/* W[k][idx[t]] = scale[k] sinh(mu[k] + sd[k] Yh[t][k]) (FP64 rows). */
+int crs_mlp_decode(const float *Yh, int ldy, const int32_t *idx, int64_t n,
+                   int64_t ld, int nsp, const double *scale, const float *mu,
+                   const float *sd, double *W, void *stream);
+
+/* dY = 2 (Yh - Yt) / (n nv) as bf16 [n][ldd] (zero beyond nv); adds the
+ * sum of squared errors to *loss when loss is not NULL. */
+int crs_mlp_mse_grad(const float *Yh, const float *Yt, int64_t n, int nv,
+                     int ldy, int ldt, void *dY, int ldd, float *loss,
+                     void *stream);
+
+/* One Adam step (bias-corrected, decoupled weight decay wd) on n FP32
+ * parameters; refreshes the BF16 copy pb. */
+int crs_mlp_adam(float *p, const float *g, float *m1, float *m2, void *pb,
+                 int64_t n, float lr, float b1, float b2, float eps, int step,
+                 float wd, void *stream);
+
+/* b = bf16(a). */
+int crs_mlp_to_bf16(const float *a, void *b, int64_t n, void *stream);
+
+/* Per-state fallback decision; flagged state indices appended to fb_idx,
+ * their count added to *fb_count (zero it first).  elem [n_elem][nsp]
+ * element counts, mw [nsp] molecular weights. */
+int crs_mlp_gate(const double *T, const double *P, const double *Y,
+                 const double *W, int64_t n, int64_t ld, int nsp,
+                 crs_mlp_gate_params gp, const double *elem, int n_elem,
+                 const double *mw, int32_t *flags, int32_t *fb_idx,
+                 int32_t *fb_count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRS_MLP_H */

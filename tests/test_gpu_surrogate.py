@@ -1,0 +1,135 @@
+"""Surrogate kernels and pipeline on the GPU.
+
+* the BF16-MFMA forward product and the full backward pass against the
+  NumPy reference (oracle/mlp_ref.py) with the same BF16 roundings;
+* training by our kernels lowers the loss;
+* gated evaluation: every state the gate sends back is bit-identical to the
+  exact FP64 kernel, out-of-box and invalid states are always sent back;
+* checkpoint round trip.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from create_rate_subs_amd import states
+from create_rate_subs_amd.surrogate import Surrogate, load_mlp, _p, _s
+from oracle import mlp_ref
+
+from _helpers import hip_lib, mechanism
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(name='mech13', seed=3):
+    specs, reacs, _ = mechanism(name)
+    return Surrogate(hip_lib(name), specs, reacs, hidden=256, seed=seed)
+
+
+def _host_params(net):
+    W, b = [], []
+    for li in range(len(net.shapes)):
+        W.append(net.w(li).float().cpu().numpy().astype(np.float64))
+        b.append(net.b(li).cpu().numpy().astype(np.float64))
+    return W, b
+
+
+@pytest.mark.parametrize('M', [1, 100, 300])
+def test_forward_matches_reference(M):
+    net = _net()
+    rng = np.random.default_rng(M)
+    X = mlp_ref.bf16(rng.normal(size=(M, net.kin)))
+    X[:, net.nin:] = 0.0
+    Xd = torch.tensor(X, dtype=torch.bfloat16, device='cuda')
+    # perturb biases so they matter
+    net.p.add_(torch.randn_like(net.p) * 0.01)
+    lib = load_mlp()
+    lib('crs_mlp_to_bf16', _p(net.p), _p(net.pb), net.nparam, _s())
+    Yh = net.forward(Xd).cpu().numpy()[:, :net.nout].astype(np.float64)
+    W, b = _host_params(net)
+    Yt = np.zeros((M, net.kout))
+    ref, _dW, _db = mlp_ref.forward_backward(X, W, b, Yt, net.nout)
+    err = np.abs(Yh - ref[:, :net.nout]).max() / np.abs(ref).max()
+    assert err < 5e-3, err
+
+
+@pytest.mark.parametrize('M', [64, 300])
+def test_gradients_match_cpu_oracle(M):
+    net = _net(seed=5)
+    rng = np.random.default_rng(7 + M)
+    X = mlp_ref.bf16(rng.normal(size=(M, net.kin)))
+    X[:, net.nin:] = 0.0
+    Ytgt = rng.normal(size=(M, net.nout)).astype(np.float32)
+    Xd = torch.tensor(X, dtype=torch.bfloat16, device='cuda')
+    Yd = torch.tensor(Ytgt, device='cuda')
+    Yh = net.forward(Xd, save=True)
+    net.backward(Yh, Yd)
+    torch.cuda.synchronize()
+    W, b = _host_params(net)
+    Yt = np.zeros((M, net.kout))
+    Yt[:, :net.nout] = Ytgt
+    _y, dW, db = mlp_ref.forward_backward(X, W, b, Yt, net.nout)
+    for li in range(len(net.shapes)):
+        n_out, n_in = net.shapes[li]
+        gw = net.gw(li).view(n_out, n_in).cpu().numpy().astype(np.float64)
+        gb = net.gb(li).cpu().numpy().astype(np.float64)
+        ew = np.linalg.norm(gw - dW[li]) / max(np.linalg.norm(dW[li]), 1e-30)
+        eb = np.linalg.norm(gb - db[li]) / max(np.linalg.norm(db[li]), 1e-30)
+        print('layer', li, 'rel dW', ew, 'rel db', eb)
+        assert ew < 2e-2 and eb < 2e-2, (li, ew, eb)
+
+
+def _labelled(name, n, seed):
+    specs, _r, _f = mechanism(name)
+    T, P, Y = states.random_states(len(specs), n, seed)
+    Td, Pd, Yd = (torch.from_numpy(a).cuda() for a in (T, P, Y))
+    W = hip_lib(name).eval_wdot(Td, Pd, Yd)
+    return Td, Pd, Yd, W
+
+
+def test_training_lowers_loss():
+    net = _net(seed=11)
+    Td, Pd, Yd, W = _labelled('mech13', 1 << 15, 50)
+    hist = net.train(Td, Pd, Yd, W, epochs=6, batch=4096, lr=2e-3)
+    print('mse by epoch', hist)
+    assert hist[-1] < 0.25 * hist[0]
+
+
+def test_gate_fallback_is_exact_and_catches_invalid_states():
+    name = 'mech13'
+    net = _net(seed=13)
+    Td, Pd, Yd, W = _labelled(name, 1 << 14, 51)
+    net.train(Td, Pd, Yd, W, epochs=2, batch=4096)
+    net.gate.rmax = 1e-3
+    T2, P2, Y2, Wex = _labelled(name, 5000, 52)
+    T2[7] = 5000.0                    # outside the training box
+    Y2[0, 9] = float('nan')           # invalid composition
+    Y2[1, 11] = -0.1                  # invalid composition
+    lib = hip_lib(name)
+    Wex = lib.eval_wdot(T2, P2, Y2)
+    Wout = torch.empty_like(Wex)
+    m = net.evaluate(T2, P2, Y2, Wout)
+    flags = net._bufs['flags'][:5000].cpu().numpy()
+    assert flags[7] and flags[9] and flags[11]
+    assert m == int(flags.sum())
+    fb = np.nonzero(flags)[0]
+    a = Wout.cpu().numpy()[:, fb]
+    b = Wex.cpu().numpy()[:, fb]
+    assert np.array_equal(a, b, equal_nan=True)
+    audit = net.audit(T2, P2, Y2, Wout, frac=0.05)
+    print('fallback', m, 'audit', audit)
+    assert audit['states'] == 250
+
+
+def test_checkpoint_roundtrip(tmp_path):
+    net = _net(seed=17)
+    Td, Pd, Yd, W = _labelled('mech13', 1 << 13, 53)
+    net.train(Td, Pd, Yd, W, epochs=1, batch=4096)
+    net.gate.rmax = 0.123
+    net.save(str(tmp_path))
+    other = _net(seed=99)
+    other.load(str(tmp_path))
+    a = net.predict(Td, Pd, Yd, torch.empty_like(W))
+    b = other.predict(Td, Pd, Yd, torch.empty_like(W))
+    assert torch.equal(a, b)
+    assert other.gate.rmax == 0.123 and other.step == net.step
