@@ -42,9 +42,19 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--no-surrogate', action='store_true')
+    ap.add_argument('--no-large', action='store_true',
+                    help='skip the 111-species / 784-reaction workload')
     ap.add_argument('--sur-epochs', type=int, default=30)
     ap.add_argument('--sur-target', type=float, default=1e-2)
+    ap.add_argument('--probe', action='store_true',
+                    help='one FP64 and one mixed launch plus a plain copy of '
+                         'known size, for hardware-counter runs')
     return ap.parse_args()
+
+
+def log(msg):
+    """Progress of the run (stderr; the result line alone goes to stdout)."""
+    print('[bench] ' + msg, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(name, T, P, Y, sample, threads):
@@ -116,6 +126,40 @@ def err_summary(torch, W, ref, S):
     return s
 
 
+def large_section(args, torch):
+    """111 species / 784 reactions, 2^20 states: FP64 and mixed paths."""
+    from create_rate_subs_amd import build, cost, runtime, states, workloads
+    from create_rate_subs_amd.create_rate_subs import load_mechanism
+    w3 = workloads.WORKLOADS['w3_111']
+    mpath, tpath = workloads.mechanism_files(w3.mech)
+    _e, specs, reacs, _f = load_mechanism(mpath, tpath)
+    lib = runtime.load(build.lib_path(w3.mech))
+    nsp = len(specs)
+    T, P, Y = states.random_states(nsp, w3.n_states, w3.state_seed)
+    Td, Pd, Yd = (torch.from_numpy(a).cuda() for a in (T, P, Y))
+    W64 = torch.empty_like(Yd)
+    Wmx = torch.empty_like(Yd)
+    steps = max(3, args.steps // 2)
+    f64_ms, f64_k = timed(torch, lambda: lib.eval_wdot(
+        Td, Pd, Yd, 'fp64', out=W64), steps, 1)
+    out = dict(workload='w3_111: {} sp / {} rxn, {} states'.format(
+        nsp, len(reacs), w3.n_states),
+        fp64_states_per_s=w3.n_states / (f64_ms * 1e-3), fp64_kernel_ms=f64_k,
+        flops_per_state=cost.flops_for(specs, reacs)['total'],
+        bytes_per_state=cost.bytes_per_state(nsp))
+    out['fp64_achieved_tflops'] = (out['flops_per_state'] * w3.n_states /
+                                   (f64_k * 1e-3) / 1e12)
+    if lib.modes & 2:
+        mx_ms, mx_k = timed(torch, lambda: lib.eval_wdot(
+            Td, Pd, Yd, 'mixed', out=Wmx), steps, 1)
+        S = gross_scale_dev(torch, lib, specs, reacs, Td, Pd, Yd,
+                            chunk=1 << 18)
+        out.update(mixed_states_per_s=w3.n_states / (mx_ms * 1e-3),
+                   mixed_kernel_ms=mx_k, mixed_speedup_vs_fp64=f64_k / mx_k,
+                   mixed_err_vs_fp64=err_summary(torch, Wmx, W64, S))
+    return out
+
+
 def surrogate_section(args, torch, lib, specs, reacs):
     """Train on 2^22 exact-labelled states, calibrate the gate on held-out
     states, then time gated inference with exact fallback on fresh states."""
@@ -135,6 +179,8 @@ def surrogate_section(args, torch, lib, specs, reacs):
     net = Surrogate(lib, specs, reacs, seed=w4.state_seed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    log('surrogate: training {} epochs on {} states'.format(args.sur_epochs,
+                                                             n))
     hist = net.train(Tt, Pt, Yt, Wt, epochs=args.sur_epochs, batch=16384)
     torch.cuda.synchronize()
     ttrain = time.perf_counter() - t0
@@ -206,10 +252,24 @@ def main():
     Wd = torch.empty_like(Yd)
     torch.cuda.synchronize()
 
+    if args.probe:
+        # counter calibration: a device copy of exactly 256 MiB read and
+        # written, next to one launch of each fused kernel on the workload
+        src = torch.ones(1 << 25, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd)
+        if lib.modes & 2:
+            lib.eval_wdot(Td, Pd, Yd, 'mixed', out=Wd)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(probe=True, states=n, copy_bytes=8 << 25)))
+        return
+
     ms_per_step, kern_ms = timed(
         torch, lambda: lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd),
         args.steps, args.warmup)
     value = n / (ms_per_step * 1e-3)
+    log('fp64 {}: {:.3e} states/s'.format(wl.name, value))
 
     fl = cost.flops_for(specs, reacs)['total']
     by = cost.bytes_per_state(nsp)
@@ -240,6 +300,8 @@ def main():
             'OMP_NUM_THREADS', min(16, os.cpu_count() or 1)))
         line['cpu_baseline'] = cpu_baseline(wl.mech, T, P, Y,
                                             args.cpu_sample, threads)
+        log('cpu baseline: {:.3e} states/s'.format(
+            line['cpu_baseline']['value']))
     del Td, Pd, Yd, Wd
 
     if not args.no_mixed and (lib.modes & 2):
@@ -274,6 +336,12 @@ def main():
         mixed['near_eq'] = dict(states=w2.near_eq_states,
                                 err_vs_fp64=err_summary(torch, Wemx, We64, Se))
         line['mixed'] = mixed
+        log('mixed: {:.3e} states/s, speed-up {:.2f}'.format(
+            mixed['states_per_s'], mixed['speedup_vs_fp64']))
+    if not args.no_large:
+        line['large'] = large_section(args, torch)
+        log('large: {:.3e} states/s fp64'.format(
+            line['large']['fp64_states_per_s']))
     if not args.no_surrogate:
         line['surrogate'] = surrogate_section(args, torch, lib, specs, reacs)
     print(json.dumps(line), flush=True)

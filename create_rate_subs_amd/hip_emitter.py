@@ -30,6 +30,7 @@ subtraction and the species sums in FP64.
 
 import math
 import os
+import re
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
 
@@ -681,7 +682,7 @@ def _species_kc_ok(x):
             all(float(nu).is_integer() for _k, nu in x.spec))
 
 
-def _species_kc_inverse(x, store_inverse=False):
+def _species_kc_inverse(x, store_inverse=False, gname=None):
     """1/Kc from per-species G_k = exp(S_k/R - H_k/RT) and (RU T / PA)^dnu.
 
     Kc = prod_k G_k^nu_k (PA/(RU T))^dnu with net coefficients nu_k; the
@@ -693,10 +694,11 @@ def _species_kc_inverse(x, store_inverse=False):
     for k, nu in x.spec:
         n = int(abs(nu))
         dnu += int(nu)
+        g = gname(k) if gname else 'G{}'.format(k)
         if nu > 0:
-            den += ['G{}'.format(k)] * n
+            den += [g] * n
         else:
-            num += ['G{}'.format(k)] * n
+            num += [g] * n
     if store_inverse:
         num += ['Gi' + d[1:] for d in den]
         den = []
@@ -708,7 +710,16 @@ def _species_kc_inverse(x, store_inverse=False):
     return expr
 
 
-def _species_gibbs_lines(ir, K, tmids, store_inverse=False):
+def _gibbs_species(ir):
+    """Species whose G_k is needed by some species-level Kc, in order."""
+    need = set()
+    for x in ir.rxns:
+        if _species_kc_ok(x):
+            need.update(k for k, _nu in x.spec)
+    return sorted(need)
+
+
+def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False):
     """Per-state G_k = exp(S_k/R - H_k/RT) and its inverse, once per species.
 
     With them Kc = prod_k G_k^nu_k (PA / (RU T))^dnu for every reversible
@@ -738,6 +749,15 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False):
         expr = ' * '.join([base] * abs(d))
         out.append('const double cpow{}{} = {};'.format(
             'p' if d > 0 else 'm', abs(d), expr))
+    if lds:
+        # one row per species, one column per lane: conflict-free access
+        # volatile keeps every access a real LDS access: the compiler would
+        # otherwise forward the stores into registers and drop the array
+        out.append('__shared__ double sG_[{}][crs::kBlock];'.format(
+            len(_gibbs_species(ir))))
+        out.append('volatile double (*sG)[crs::kBlock] = sG_;')
+        out.append('const int lx = threadIdx.x;')
+    slots = {k: s for s, k in enumerate(_gibbs_species(ir))}
     for k in sorted(need_g | need_gi):
         sp = ir.nasa[k]
         gi = tmids.index(sp[0]) if sp[0] in tmids else None
@@ -749,9 +769,14 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False):
                 if cv != 0.0:
                     terms.append('{} * p{}{}_{}'.format(K(cv), half, gi, bi))
         out.append('const double smh{} = {};'.format(k, ' + '.join(terms)))
+        if lds:
+            out.append('sG[{s}][lx] = exp(smh{k});'.format(s=slots[k], k=k))
+            continue
         out.append('const double G{k} = exp(smh{k});'.format(k=k))
         if k in need_gi:
             out.append('const double Gi{k} = 1.0 / G{k};'.format(k=k))
+    if lds == 'sync':
+        out.append('__syncthreads();')
     return out
 
 
@@ -855,6 +880,29 @@ def _mixed_pm_lines(x, K, kf_x):
     return out
 
 
+_C_REF = re.compile(r'\bC\[(\d+)\]')
+
+
+def _concentrations_to_lds(lines, nsp):
+    """Keep the per-state concentrations in LDS instead of registers.
+
+    One [NSP][block] FP64 array, one column per lane (conflict-free), read
+    through a volatile pointer so every use is an LDS load rather than a
+    register the compiler keeps live across the whole reaction list.
+    """
+    out = []
+    for l in lines:
+        if l == 'double C[{}];'.format(nsp):
+            out.append('__shared__ double sC[{}][crs::kBlock];'.format(nsp))
+            out.append('const int lxc = threadIdx.x;')
+            continue
+        if l.startswith('const double logT = log(T);'):
+            # concentrations complete: publish them, then read back
+            out.append('__syncthreads();')
+        out.append(_C_REF.sub(r'sC[\1][lxc]', l))
+    return out
+
+
 def _fused_fast_lines(ir, K, opts, mixed=False):
     """Per-state body of the fused FP64 kernel, arranged for the GPU.
 
@@ -907,7 +955,12 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
     species_kc = opts.get('kc') == 'species'
     if species_kc:
         lines += _species_gibbs_lines(ir, K, tmids,
-                                      opts.get('kc_inverse') == 'store')
+                                      opts.get('kc_inverse') == 'store',
+                                      {'lds': True, 'lds_sync': 'sync'}.get(
+                                          opts.get('g_store'), False))
+    gslots = {k: s for s, k in enumerate(_gibbs_species(ir))}
+    gname = ((lambda k: 'sG[{}][lx]'.format(gslots[k]))
+             if opts.get('g_store') in ('lds', 'lds_sync') else None)
     last = _last_touch(ir)
     seen = [False] * nsp
     lines.append('double w[{}];'.format(nsp))
@@ -928,7 +981,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                 # concentration product cannot underflow prematurely
                 inner.append('const double r = {}(kf * {});'.format(
                     _conc_prod(x.prod), _species_kc_inverse(
-                        x, opts.get('kc_inverse') == 'store')))
+                        x, opts.get('kc_inverse') == 'store', gname)))
             else:
                 # remap this reaction's Tmid groups onto the global masks
                 groups = []
@@ -981,6 +1034,8 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                 lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
+    if opts.get('c_store') == 'lds':
+        lines = _concentrations_to_lds(lines, ir.nsp)
     return lines
 
 
