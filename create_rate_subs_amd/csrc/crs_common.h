@@ -82,6 +82,47 @@ __device__ __forceinline__ double rcp64(double x)
     return __builtin_fma(r, e, r);
 }
 
+// e^x in FP64 to about 1.2 ulp (max relative error 1.19 * 2^-53 measured by
+// exact emulation of the FMA sequence).  Cody-Waite reduction
+// x = n ln2 + r, |r| <= ln2/2, with ln2 split into a double and its
+// remainder; e^r = 1 + r (1 + r q(r)) where q(r) = (e^r - 1 - r) / r^2 is
+// the degree-9 Chebyshev interpolant on [-ln2/2, ln2/2] (coefficients from
+// exp64_coeffs.py beside this file); the result is scaled by 2^n with
+// ldexp, which also delivers overflow to +inf and underflow to 0.  It takes
+// no branch and no range test, unlike the library exp: finite arguments and
+// NaN only (exp(-inf) would give NaN, which no rate expression produces).
+__device__ __forceinline__ double exp64(double x)
+{
+    const double n = __builtin_rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-n, 0.6931471805599453, x);
+    r = __builtin_fma(-n, 2.3190468138462996e-17, r);
+    double q = 2.510038549551032e-08;
+    q = __builtin_fma(q, r, 2.7620088445409746e-07);
+    q = __builtin_fma(q, r, 2.7557268459997064e-06);
+    q = __builtin_fma(q, r, 2.4801521295954376e-05);
+    q = __builtin_fma(q, r, 0.00019841269863053618);
+    q = __builtin_fma(q, r, 0.0013888888917213717);
+    q = __builtin_fma(q, r, 0.008333333333330062);
+    q = __builtin_fma(q, r, 0.04166666666662413);
+    q = __builtin_fma(q, r, 0.16666666666666669);
+    q = __builtin_fma(q, r, 0.5000000000000001);
+    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
+// Per-lane LDS storage.  A copy of an LDS pointer taken through an empty asm
+// is a value the compiler cannot see through: loads made through it can be
+// neither hoisted above the point where the copy is taken nor forwarded from
+// the stores that filled the array, so data parked in LDS stays there until
+// the code that needs it.
+#define CRS_LDS __attribute__((address_space(3)))
+
+__device__ __forceinline__ CRS_LDS double *opaque_lds(CRS_LDS double *p)
+{
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
 // ---------------------------------------------------------------------------
 // Mixed-precision building blocks.  The argument of an exponential is formed
 // and range-reduced in FP64 (x log2 e = n + f, |f| <= 1/2, exact since

@@ -1,0 +1,74 @@
+"""Coefficients of crs::exp64 (crs_common.h) and their accuracy check.
+
+e^r = 1 + r (1 + r q(r)) on |r| <= ln2/2, with q(r) = (e^r - 1 - r) / r^2
+approximated by its degree-9 Chebyshev interpolant (mpmath.chebyfit, 60
+digits).  The check emulates the device sequence exactly: every FMA is
+rounded once to double, the reduction uses ln2 split into a double and its
+remainder.  Run: python exp64_coeffs.py [samples]
+"""
+
+import random
+import sys
+
+import mpmath as mp
+
+LOG2E = 1.4426950408889634
+
+
+def coefficients(degree=9):
+    """q's coefficients, highest power first, rounded to double."""
+    mp.mp.dps = 60
+    a = mp.log(2) / 2
+
+    def q(r):
+        if abs(r) < mp.mpf('1e-20'):
+            return mp.mpf(1) / 2
+        return (mp.exp(r) - 1 - r) / r ** 2
+
+    poly, _err = mp.chebyfit(q, [-a * 1.0001, a * 1.0001], degree + 1,
+                             error=True)
+    return [float(c) for c in poly]
+
+
+def ln2_split():
+    mp.mp.dps = 60
+    hi = float(mp.log(2))
+    return hi, float(mp.log(2) - mp.mpf(hi))
+
+
+def emulate(x, cs, hi, lo):
+    """The device sequence with each operation rounded to double."""
+    def rd(v):
+        return mp.mpf(float(v))
+
+    def fma(a, b, c):
+        return rd(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+    n = mp.nint(rd(mp.mpf(x) * LOG2E))
+    r = fma(-n, hi, x)
+    r = fma(-n, lo, r)
+    q = mp.mpf(cs[0])
+    for c in cs[1:]:
+        q = fma(q, r, c)
+    p = fma(r, fma(r, q, 1.0), 1.0)
+    return rd(mp.ldexp(p, int(n)))
+
+
+def max_error(samples=20000, seed=1):
+    """Max relative error of the emulated sequence, in units of 2^-53."""
+    cs = coefficients()
+    hi, lo = ln2_split()
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        x = rng.uniform(-700.0, 700.0) if i % 2 else rng.uniform(-2.0, 2.0)
+        ex = mp.exp(mp.mpf(x))
+        worst = max(worst, float(abs(emulate(x, cs, hi, lo) - ex) / ex))
+    return worst / 2.0 ** -53
+
+
+if __name__ == '__main__':
+    print('q coefficients:', [repr(c) for c in coefficients()])
+    print('ln2 split:', [repr(v) for v in ln2_split()])
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 20000
+    print('max error (units of 2^-53):', max_error(n))

@@ -500,19 +500,73 @@ def _indent(lines, n):
     return ''.join(pad + l + '\n' for l in lines)
 
 
-# Measured on MI355X (53 sp / 325 rxn, 2^20 states): immediates beat a
-# scalar-loaded table 2x at one wave per SIMD (no load latency to hide), and
-# species-level Kc removes one exponential per reversible reaction.
+# Measured on MI355X (2^20 states):
+# * immediates beat a scalar-loaded table 2x at one wave per SIMD (each
+#   s_load is waited for at once, nothing hides it);
+# * species-level Kc removes one exponential per reversible reaction;
+# * with every per-species array in registers the kernels spill (388-584
+#   bytes per lane at 53 species, 3.5-4 KB at 111).  Parking one array in
+#   per-lane LDS rows removes the spills at 53 species (FP64 1.54 -> 1.37 ms,
+#   mixed 1.07 -> 0.88 ms with the H/G net-rate form below) and cuts them
+#   by 60 % at 111 species (FP64 7.2 -> 4.2 ms, mixed 7.9 -> 3.0 ms);
+# * the retiring reaction order and the branch-free exp64 add a few %.
+# 'auto' picks the H/G form with G in LDS when every species fits the LDS
+# budget (78 rows), otherwise species Kc with 78 concentration rows in LDS.
 DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
-                    fused='fast', kc='species')
+                    fused='fast', kc='auto', exp='crs', order='retire')
 
 
-def _last_touch(ir):
+def resolve_opts(nsp, opts=None):
+    """Emitter options with 'auto' choices made for an NSP-species mechanism."""
+    o = dict(DEFAULT_OPTS, **(opts or {}))
+    if o.get('kc') == 'auto':
+        if nsp <= o.get('lds_species', 78):
+            o['kc'] = 'species_hg'
+            o.setdefault('g_store', 'lane_lds')
+        else:
+            o['kc'] = 'species'
+            o.setdefault('c_store', 'lane_lds')
+    return o
+
+
+def _last_touch(ir, rxns=None):
     last = {}
-    for x in ir.rxns:
+    for x in (ir.rxns if rxns is None else rxns):
         for k, _nu in x.spec:
             last[k] = x.index
     return last
+
+
+def _species_of(x):
+    return ({k for k, _nu in x.spec} | {k for k, _nu in x.reac} |
+            {k for k, _nu in x.prod} | {k for k, _e in x.eff} |
+            ({x.pdep_sp} if x.pdep_sp is not None else set()))
+
+
+def _retiring_order(ir):
+    """Reaction order that retires species early (fused kernels only).
+
+    Greedy: repeatedly take the species, among those already touched
+    (then any), with the fewest reactions left, and emit all of them.  A
+    species' accumulator is live from its first to its last reaction and
+    its concentration until its last use, so finishing species early cuts
+    the mean live set by about a third on the synthetic mechanisms.  The
+    sums of each species change order, nothing else.
+    """
+    left = set(range(len(ir.rxns)))
+    by_sp = {}
+    for j, x in enumerate(ir.rxns):
+        for k in _species_of(x):
+            by_sp.setdefault(k, set()).add(j)
+    order, active = [], set()
+    while left:
+        cand = [k for k in sorted(by_sp) if by_sp[k] & left]
+        k = min(cand, key=lambda k: (k not in active, len(by_sp[k] & left), k))
+        for j in sorted(by_sp[k] & left):
+            order.append(ir.rxns[j])
+            left.discard(j)
+            active |= _species_of(ir.rxns[j])
+    return order
 
 
 def _fused_f64_lines(ir, K, opts):
@@ -719,7 +773,8 @@ def _gibbs_species(ir):
     return sorted(need)
 
 
-def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False):
+def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
+                         every=False, expfn='exp'):
     """Per-state G_k = exp(S_k/R - H_k/RT) and its inverse, once per species.
 
     With them Kc = prod_k G_k^nu_k (PA / (RU T))^dnu for every reversible
@@ -738,6 +793,8 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False):
             d += int(nu)
         if d:
             dnus.add(d)
+    if every:
+        need_g = set(range(ir.nsp))
     out = []
     if not (need_g or need_gi):
         return out
@@ -772,11 +829,46 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False):
         if lds:
             out.append('sG[{s}][lx] = exp(smh{k});'.format(s=slots[k], k=k))
             continue
-        out.append('const double G{k} = exp(smh{k});'.format(k=k))
+        out.append('const double G{k} = {e}(smh{k});'.format(k=k, e=expfn))
         if k in need_gi:
             out.append('const double Gi{k} = 1.0 / G{k};'.format(k=k))
     if lds == 'sync':
         out.append('__syncthreads();')
+    return out
+
+
+def _hg_ok(x):
+    """Reaction whose rates take the H/G form: species-level Kc and integral
+    stoichiometric coefficients on both sides."""
+    return _species_kc_ok(x) and all(is_integer(nu)
+                                     for _k, nu in x.reac + x.prod)
+
+
+def _hg_net_rate(x, gref, pm):
+    """Net rate of a reversible reaction from G_k and H_k = C_k / G_k.
+
+    f = kf prod_r C_r and r = kf / Kc prod_p C_p with
+    1/Kc = prod_r G_r / prod_p G_p (RU T / PA)^dnu, hence
+    q = kf prod_r G_r (prod_r H_r - (RU T / PA)^dnu prod_p H_p):
+    no reciprocal per reaction, and the forward-minus-reverse difference
+    is one FMA.
+    """
+    def prod(terms, fmt):
+        f = []
+        for k, nu in terms:
+            f += [fmt(k)] * int(nu)
+        return ' * '.join(f) if f else '1.0'
+    dnu = sum(int(nu) for _k, nu in x.spec)
+    gr = prod(x.reac, gref)
+    hr = prod(x.reac, lambda k: 'H{}'.format(k))
+    hp = prod(x.prod, lambda k: 'H{}'.format(k))
+    out = ['const double kg = kf * ({});'.format(gr)]
+    if dnu:
+        cp = 'cpow{}{}'.format('m' if dnu < 0 else 'p', abs(dnu))
+        out.append('const double q = kg * __builtin_fma(-{}, {}, {}){};'.format(
+            cp, hp, hr, pm))
+    else:
+        out.append('const double q = kg * (({}) - ({})){};'.format(hr, hp, pm))
     return out
 
 
@@ -903,6 +995,66 @@ def _concentrations_to_lds(lines, nsp):
     return out
 
 
+def _concentrations_to_lane_lds(lines, nsp, nl):
+    """Park the concentrations of species 0..nl-1 in LDS, one slot per lane.
+
+    Each wave owns an [nl][64] FP64 array (a species row is 512 bytes, so
+    every access is a ds_read_b64 with an immediate offset and no bank
+    conflict).  Every reaction block re-reads the concentrations it needs
+    through an opaque copy of the lane's base pointer, taken at the top of
+    the block; the values are therefore live only inside the block instead
+    of across the whole reaction list.  No barrier is needed: a lane reads
+    only what it wrote itself.
+    """
+    # the nl species whose concentration is needed longest go to LDS
+    last_use = {}
+    for pos, l in enumerate(lines):
+        for mo in _C_REF.finditer(l):
+            last_use[int(mo.group(1))] = pos
+    parked = sorted(last_use, key=lambda k: (-last_use[k], k))[:nl]
+    slot = {k: s for s, k in enumerate(sorted(parked))}
+    out = []
+    published = False
+    for l in lines:
+        if not published and l.startswith('const double logT = log(T);'):
+            out.append('__shared__ double sCl[crs::kBlock / 64][{}][64];'
+                       .format(max(len(slot), 1)))
+            out.append('CRS_LDS double *const cb = (CRS_LDS double *)'
+                       '&sCl[threadIdx.x >> 6][0][threadIdx.x & 63];')
+            out += ['cb[{s} * 64] = C[{k}];'.format(s=s, k=k)
+                    for k, s in sorted(slot.items())]
+            published = True
+            out.append(l)
+            continue
+        if not published:
+            out.append(l)
+            continue
+        l = _C_REF.sub(lambda mo: ('cl[{} * 64]'.format(slot[int(mo.group(1))])
+                                   if int(mo.group(1)) in slot
+                                   else mo.group(0)), l)
+        out.append(l)
+        if l.startswith('{  // rxn'):
+            out.append('    CRS_LDS double *const cl = crs::opaque_lds(cb);')
+    return out
+
+
+def _species_updates(x, seen):
+    """w_k +/-= |nu| q for the species of reaction x (first touch assigns)."""
+    out = []
+    for k, nu in x.spec:
+        anu = abs(nu)
+        term = 'q' if anu == 1.0 else '{} * q'.format(
+            _lit(emitted(py2_str(float(anu)))))
+        if seen[k]:
+            out.append('w[{k}] = w[{k}] {op} {t};'.format(
+                k=k, op='-' if nu < 0 else '+', t=term))
+        else:
+            out.append('w[{k}] = {n}{t};'.format(
+                k=k, n='-' if nu < 0 else '', t=term))
+            seen[k] = True
+    return out
+
+
 def _fused_fast_lines(ir, K, opts, mixed=False):
     """Per-state body of the fused FP64 kernel, arranged for the GPU.
 
@@ -934,7 +1086,8 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
     if mixed:
         lines += ['const float mf = (float)m;', 'const float Tf = (float)T;',
                   '(void)mf; (void)Tf;']
-    expfn = 'crs::exp_mixed' if mixed else 'exp'
+    fexp = 'crs::exp64' if opts.get('exp') == 'crs' else 'exp'
+    expfn = 'crs::exp_mixed' if mixed else fexp
     tmids = []
     for x in ir.rxns:
         for tm, _lo, _hi in x.kc_groups:
@@ -952,23 +1105,57 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                 g=gi, b=bi, v=bname))
             lines.append('const double ph{g}_{b} = lo{g} ? 0.0 : {v};'.format(
                 g=gi, b=bi, v=bname))
-    species_kc = opts.get('kc') == 'species'
+    hg = opts.get('kc') == 'species_hg'
+    species_kc = opts.get('kc') in ('species', 'species_hg')
     if species_kc:
         lines += _species_gibbs_lines(ir, K, tmids,
                                       opts.get('kc_inverse') == 'store',
                                       {'lds': True, 'lds_sync': 'sync'}.get(
-                                          opts.get('g_store'), False))
+                                          opts.get('g_store'), False),
+                                      every=hg, expfn=fexp)
     gslots = {k: s for s, k in enumerate(_gibbs_species(ir))}
     gname = ((lambda k: 'sG[{}][lx]'.format(gslots[k]))
              if opts.get('g_store') in ('lds', 'lds_sync') else None)
-    last = _last_touch(ir)
+    g_lane = hg and opts.get('g_store') == 'lane_lds'
+    # LDS budget: one wave per SIMD leaves 40 KB per wave, 80 FP64 rows
+    n_gl = min(nsp, opts.get('lds_species', 78)) if g_lane else 0
+    gref = (lambda k: 'gl[{} * 64]'.format(k) if k < n_gl else
+            'G{}'.format(k))
+    if hg:
+        # H_k = C_k / G_k; from here on C_k is G_k H_k (no third array)
+        lines += ['const double H{k} = C[{k}] * crs::rcp64(G{k});'.format(k=k)
+                  for k in range(nsp)]
+        if g_lane:
+            lines.append('__shared__ double sGl[crs::kBlock / 64][{}][64];'
+                         .format(n_gl))
+            lines.append('CRS_LDS double *const gb = (CRS_LDS double *)'
+                         '&sGl[threadIdx.x >> 6][0][threadIdx.x & 63];')
+            lines += ['gb[{k} * 64] = G{k};'.format(k=k) for k in range(n_gl)]
+    rx_start = len(lines)
+    rxns = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
+    last = _last_touch(ir, rxns)
     seen = [False] * nsp
     lines.append('double w[{}];'.format(nsp))
     for k in range(nsp):
         if k not in last:
             lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
-    for x in ir.rxns:
+    for x in rxns:
         inner, xf = _fast_arr(x.arr, K, 'kf', expfn)
+        if hg and _hg_ok(x):
+            pm = ''
+            if x.pd_index is not None:
+                inner += (_mixed_pm_lines(x, K, xf) if mixed else
+                          _fast_pm_lines(x, K, xf))
+                pm = ' * pm'
+            inner += _hg_net_rate(x, gref, pm)
+            inner += _species_updates(x, seen)
+            lines.append('{  // rxn %d' % x.index)
+            lines += ['    ' + l for l in inner]
+            lines.append('}')
+            for k, _nu in x.spec:
+                if last[k] == x.index:
+                    lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
+            continue
         inner.append('const double f = {}kf;'.format(_conc_prod(x.reac)))
         if x.rev:
             if x.rev_arr is not None:
@@ -1015,17 +1202,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                       _fast_pm_lines(x, K, xf))
             q = q + ' * pm'
         inner.append('const double q = {};'.format(q))
-        for k, nu in x.spec:
-            anu = abs(nu)
-            term = 'q' if anu == 1.0 else '{} * q'.format(
-                _lit(emitted(py2_str(float(anu)))))
-            if seen[k]:
-                inner.append('w[{k}] = w[{k}] {op} {t};'.format(
-                    k=k, op='-' if nu < 0 else '+', t=term))
-            else:
-                inner.append('w[{k}] = {n}{t};'.format(
-                    k=k, n='-' if nu < 0 else '', t=term))
-                seen[k] = True
+        inner += _species_updates(x, seen)
         lines.append('{  // rxn %d' % x.index)
         lines += ['    ' + l for l in inner]
         lines.append('}')
@@ -1034,14 +1211,32 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                 lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
+    if hg:
+        # concentrations after the H_k lines: C_k = G_k * H_k
+        def c_of(mo):
+            k = int(mo.group(1))
+            return '({} * H{})'.format(gref(k), k)
+        rx = [_C_REF.sub(c_of, l) for l in lines[rx_start:]]
+        if g_lane:
+            out = []
+            for l in rx:
+                out.append(l)
+                if l.startswith('{  // rxn'):
+                    out.append('    CRS_LDS double *const gl = '
+                               'crs::opaque_lds(gb);')
+            rx = out
+        lines = lines[:rx_start] + rx
     if opts.get('c_store') == 'lds':
         lines = _concentrations_to_lds(lines, ir.nsp)
+    elif opts.get('c_store') == 'lane_lds':
+        lines = _concentrations_to_lane_lds(
+            lines, ir.nsp, min(ir.nsp, opts.get('lds_species', 78)))
     return lines
 
 
 def emit_hip_source(ir, opts=None):
     """Full HIP source text for mechanism ``ir``."""
-    opts = dict(DEFAULT_OPTS, **(opts or {}))
+    opts = resolve_opts(ir.nsp, opts)
     K = _Consts(opts['consts'])
     nsp, nr, nrev, npd = ir.nsp, ir.nr, ir.nrev, ir.npd
     ru = _lit(emitted(fmt_e8(chem.RU)))

@@ -1,0 +1,43 @@
+"""crs::exp64 (the branch-free FP64 exponential of the fused kernels).
+
+The coefficients in crs_common.h must be the ones exp64_coeffs.py derives
+(Chebyshev interpolant of (e^r - 1 - r)/r^2), and the device sequence,
+emulated with one rounding per FMA, must stay within 1.5 units of 2^-53
+relative of e^x (mpmath, 60 digits) over |x| <= 700.
+"""
+
+import importlib.util
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, 'create_rate_subs_amd', 'csrc')
+
+
+def _coeffs_module():
+    spec = importlib.util.spec_from_file_location(
+        'exp64_coeffs', os.path.join(CSRC, 'exp64_coeffs.py'))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _device_constants():
+    text = open(os.path.join(CSRC, 'crs_common.h')).read()
+    body = text[text.index('double exp64(double x)'):]
+    body = body[:body.index('\n}\n')]
+    return [float(v) for v in re.findall(r'(-?\d+\.\d+(?:e-?\d+)?)', body)]
+
+
+def test_header_holds_derived_coefficients():
+    mod = _coeffs_module()
+    consts = _device_constants()
+    hi, lo = mod.ln2_split()
+    assert consts[0] == 1.4426950408889634
+    assert consts[1:3] == [hi, lo]
+    assert consts[3:13] == mod.coefficients()
+
+
+def test_emulated_sequence_error_bound():
+    mod = _coeffs_module()
+    assert mod.max_error(samples=3000, seed=7) <= 1.5
