@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0      # dense BF16 MFMA, MI355X
 HBM_PEAK_GBS = 8000.0
 TAU = 5e-2
 
@@ -160,10 +161,17 @@ def large_section(args, torch):
     return out
 
 
+def _quantiles(torch, x, qs=(0.01, 0.1, 0.5, 0.9, 0.99)):
+    """Quantiles of a per-state error vector (strided sample if large)."""
+    x = x[::max(1, x.numel() >> 20)].double()
+    x = torch.where(torch.isfinite(x), x, torch.full_like(x, float('inf')))
+    return {str(q): x.quantile(q).item() for q in qs}
+
+
 def surrogate_section(args, torch, lib, specs, reacs):
     """Train on 2^22 exact-labelled states, calibrate the gate on held-out
     states, then time gated inference with exact fallback on fresh states."""
-    from create_rate_subs_amd import metrics, states, workloads
+    from create_rate_subs_amd import cost, metrics, states, workloads
     from create_rate_subs_amd.surrogate import Surrogate
     w4 = workloads.WORKLOADS['w4_53']
     nsp = len(specs)
@@ -191,11 +199,14 @@ def surrogate_section(args, torch, lib, specs, reacs):
     # gate calibration on held-out states
     Tv, Pv, Yv, Wv = labelled(1 << 18, w4.state_seed + 100)
     Sv = gross_scale_dev(torch, lib, specs, reacs, Tv, Pv, Yv)
+    _r, _a, curve_nw = net.calibrate_gate(Tv, Pv, Yv, Wv, Sv, args.sur_target,
+                                          TAU, metric='normwise')
     rmax, admitted, curve = net.calibrate_gate(Tv, Pv, Yv, Wv, Sv,
                                                args.sur_target, TAU)
     out['gate'] = dict(target=args.sur_target, tau=TAU, rmax=rmax,
                        validation_admitted=admitted,
-                       tradeoff_on_validation=curve)
+                       tradeoff_on_validation=curve,
+                       tradeoff_normwise_on_validation=curve_nw)
     del Tv, Pv, Yv, Wv, Sv
     # fresh test states
     Te, Pe, Ye, We = labelled(n, w4.state_seed + 200)
@@ -203,8 +214,8 @@ def surrogate_section(args, torch, lib, specs, reacs):
     steps = 3
     f64_ms, _k = timed(torch, lambda: lib.eval_wdot(Te, Pe, Ye, 'fp64',
                                                      out=Wout), steps, 1)
-    pred_ms, _k = timed(torch, lambda: net.predict(Te, Pe, Ye, Wout),
-                        steps, 1)
+    pred_ms, pred_kern_ms = timed(
+        torch, lambda: net.predict(Te, Pe, Ye, Wout), steps, 1)
     fb = []
 
     def gated():
@@ -213,6 +224,29 @@ def surrogate_section(args, torch, lib, specs, reacs):
     S = gross_scale_dev(torch, lib, specs, reacs, Te, Pe, Ye)
     e = metrics.rel_err(Wout, We, S, TAU)
     per_state = e.max(dim=0).values
+    nw = metrics.normwise_err(Wout, We)
+    # the network alone, before the gate (fused kernel, all states)
+    Wsur = torch.empty_like(We)
+    net.predict(Te, Pe, Ye, Wsur)
+    e_raw = metrics.rel_err(Wsur, We, S, TAU).max(dim=0).values
+    nw_raw = metrics.normwise_err(Wsur, We)
+    del Wsur
+    flops = cost.surrogate_flops_per_state(net.nin, net.hid,
+                                           len(net.shapes) - 1, net.nout)
+    achieved = flops * n / (pred_ms * 1e-3) / 1e12
+    out['kernel'] = dict(
+        name='k_mlp_infer', ms=pred_ms, flops_per_state=flops,
+        achieved_tflops=achieved, peak_tflops=BF16_PEAK_TFLOPS,
+        frac=achieved / BF16_PEAK_TFLOPS,
+        note='fused features + 55-256-256-256-53 GELU MLP (BF16 MFMA, FP32 '
+             'accumulate) + decode + gate, one launch; flops are the '
+             'unpadded 2*MACs')
+    out['network_alone'] = dict(
+        conditioned_state_quantiles=_quantiles(torch, e_raw),
+        normwise_state_quantiles=_quantiles(torch, nw_raw),
+        frac_conditioned_le_target=(e_raw <= args.sur_target).double().mean().item(),
+        frac_normwise_le_target=(nw_raw <= args.sur_target).double().mean().item(),
+        frac_normwise_le_0p1=(nw_raw <= 0.1).double().mean().item())
     out['test'] = dict(
         states=n, gated_states_per_s=n / (gated_ms * 1e-3),
         surrogate_only_states_per_s=n / (pred_ms * 1e-3),
@@ -222,7 +256,7 @@ def surrogate_section(args, torch, lib, specs, reacs):
         post_fallback_err=dict(max=e.max().item(),
                                state_median=per_state.median().item(),
                                state_p99=per_state[::16].quantile(0.99).item(),
-                               tau=TAU),
+                               tau=TAU, normwise_max=nw.max().item()),
         audit_1pct=net.audit(Te, Pe, Ye, Wout, frac=0.01))
     return out
 

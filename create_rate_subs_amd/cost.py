@@ -82,3 +82,10 @@ def bytes_per_state(nsp):
 
 def flops_for(specs, reacs):
     return flops_per_state(build_ir(specs, reacs))
+
+
+def surrogate_flops_per_state(nin, hidden, layers, nout):
+    """2 x multiply-adds of the MLP nin -> hidden^layers -> nout (unpadded):
+    317,440 for 55 -> 3 x 256 -> 53."""
+    dims = [nin] + [hidden] * layers + [nout]
+    return 2 * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))

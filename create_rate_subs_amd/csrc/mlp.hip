@@ -32,18 +32,21 @@ constexpr int LROW = TR + LDS_PAD;   // LDS row length in bf16 (80 bytes)
 __device__ __forceinline__ float bf2f(__bf16 x) { return (float)x; }
 __device__ __forceinline__ __bf16 f2bf(float x) { return (__bf16)x; }
 
-// GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
+// GELU, tanh form: 0.5 z (1 + tanh(u)), u = sqrt(2/pi) (z + 0.044715 z^3),
+// evaluated as z sigma(2u) = z / (1 + 2^(z (k0 + k1 z^2))) with
+// k0 = -2 sqrt(2/pi) log2(e), k1 = 0.044715 k0: one hardware exp2 and one
+// hardware reciprocal.  With s = sigma(2u), tanh(u) = 2s - 1 and
+// gelu'(z) = s + 2 z s (1 - s) sqrt(2/pi) (1 + 3 * 0.044715 z^2).
 __device__ __forceinline__ float gelu_t(float z, float *dgelu)
 {
-    const float c = 0.7978845608028654f;
-    const float u = c * (z + 0.044715f * z * z * z);
-    // tanh(u) = 1 - 2 / (exp(2u) + 1), exp on the hardware exp2
-    const float e = __builtin_amdgcn_exp2f(2.0f * u * 1.4426950408889634f);
-    const float t = 1.0f - 2.0f / (e + 1.0f);
+    const float k0 = -2.3022082f, k1 = -0.10294324f;
+    const float z2 = z * z;
+    const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(k1, z2, k0));
+    const float sg = __builtin_amdgcn_rcpf(1.0f + e);
     if (dgelu)
-        *dgelu = 0.5f * (1.0f + t) +
-                 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
-    return 0.5f * z * (1.0f + t);
+        *dgelu = sg + 2.0f * z * sg * (1.0f - sg) * 0.7978845608028654f *
+                          (1.0f + 0.134145f * z2);
+    return z * sg;
 }
 
 enum Epi {
@@ -178,6 +181,15 @@ __global__ __launch_bounds__(256) void k_gemm(
     }
 }
 
+// ln of a positive value on the FP32 hardware log2 (inputs are clamped to
+// at least 1e-30, inside the FP32 normal range).  Used for every feature so
+// that training and fused inference see identical inputs.
+__device__ __forceinline__ float ln_feature(double v)
+{
+    const float x = (float)(v > 1e-30 ? v : 1e-30);
+    return __builtin_amdgcn_logf(x) * 0.69314718055994531f;
+}
+
 // Features: x = [(ln T - mu0)/sd0, (ln P - mu1)/sd1, (ln max(Y_k,1e-30) - mu)/sd]
 // for k < nsp (the last species is 1 - sum of the others), padded with zeros.
 __global__ __launch_bounds__(256) void k_features(
@@ -191,18 +203,15 @@ __global__ __launch_bounds__(256) void k_features(
         return;
     const int64_t i = idx ? (int64_t)idx[t] : t;
     __bf16 *x = X + t * kp;
-    x[0] = f2bf(((float)log(T[i]) - mu[0]) / sd[0]);
-    x[1] = f2bf(((float)log(P[i]) - mu[1]) / sd[1]);
+    x[0] = f2bf((ln_feature(T[i]) - mu[0]) / sd[0]);
+    x[1] = f2bf((ln_feature(P[i]) - mu[1]) / sd[1]);
     double s = 0.0;
     for (int k = 0; k < nsp - 1; ++k) {
         const double y = Y[(int64_t)k * ld + i];
         s += y;
-        const float ly = (float)log(y > 1e-30 ? y : 1e-30);
-        x[2 + k] = f2bf((ly - mu[2 + k]) / sd[2 + k]);
+        x[2 + k] = f2bf((ln_feature(y) - mu[2 + k]) / sd[2 + k]);
     }
-    const double yN = 1.0 - s;
-    const float lyN = (float)log(yN > 1e-30 ? yN : 1e-30);
-    x[1 + nsp] = f2bf((lyN - mu[1 + nsp]) / sd[1 + nsp]);
+    x[1 + nsp] = f2bf((ln_feature(1.0 - s) - mu[1 + nsp]) / sd[1 + nsp]);
     for (int c = 2 + nsp; c < kp; ++c)
         x[c] = f2bf(0.0f);
 }
@@ -365,6 +374,403 @@ __global__ __launch_bounds__(256) void k_gate(
     if (bad) {
         const int32_t slot = atomicAdd(fb_count, 1);
         fb_idx[slot] = (int32_t)i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: features -> 3 x (256, GELU) -> 64 outputs -> decode -> gate
+// in one kernel, activations never leaving registers.
+//
+// A wave owns 64 states as four 16-state column blocks.  Every product is
+// D = W_tile (16 x 32, from LDS) x act (32 x 16 states) on
+// v_mfma_f32_16x16x32_bf16, so a layer's output lands with lane l holding
+// units 16r + 4(l/16) + t (t < 4) of state l % 16.  Two consecutive row
+// blocks hold exactly the 8 values lane l must supply as the next layer's
+// B operand for one 32-deep step, provided the next layer's reduction index
+// is permuted the same way: step s, lane group g, element e reads unit
+// 32 s + 16 (e / 4) + 4 g + e % 4.  crs_mlp_pack writes every weight matrix
+// in that tile order (64 lanes x 8 bf16 per 16 x 32 tile), so activations
+// pass from layer to layer with no shuffle and no LDS round trip.
+//
+// The block (4 waves, 256 states) stages one layer's tiles at a time in
+// 128 KB of LDS; each tile read feeds four MFMAs (one per column block).
+// Features go through LDS once (per-lane state -> B layout), outputs once
+// (D layout -> per-lane state), then each lane decodes its state's rates to
+// FP64, applies the gate and appends fallback states with one atomic per
+// wave.
+// ---------------------------------------------------------------------------
+constexpr int IH = 256;           // hidden width
+constexpr int IKIN = 64;          // padded input width
+constexpr int IKOUT = 64;         // padded output width
+constexpr int INC = 2;            // 16-state column blocks per wave
+constexpr int IWAVES = 8;         // two per SIMD: one multiplies while
+                                  // the other runs its GELU epilogue
+constexpr int ITHREADS = 64 * IWAVES;
+constexpr int ISTATES = 16 * INC * IWAVES;
+constexpr int IFEAT_LD = 72;      // bf16 per state row (144 B, 16 B aligned)
+constexpr int IOUT_LD = 68;       // floats per state row
+constexpr int ILDS = 128 * 1024;  // one hidden layer of tiles
+
+static_assert(2 * ISTATES == ITHREADS, "two threads per state in the state stages");
+static_assert(ISTATES * IFEAT_LD * 2 + ITHREADS * 12 <= ILDS,
+              "feature stage and its exchange fit");
+static_assert(ISTATES * IOUT_LD * 4 <= ILDS, "output stage fits");
+static_assert(IH * IH * 2 <= ILDS, "hidden layer fits");
+
+// Tile element (lane l = i + 16 g, element e) of row block r, step s.
+__device__ __forceinline__ int tile_k(int perm, int s, int g, int e)
+{
+    return perm ? 32 * s + 16 * (e >> 2) + 4 * g + (e & 3)
+                : 32 * s + 8 * g + e;
+}
+
+__global__ __launch_bounds__(256) void k_pack_tiles(
+    const __bf16 *__restrict__ W, int n_out, int n_in, int perm,
+    __bf16 *__restrict__ out)
+{
+    const int ns = n_in / 32;
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)(n_out / 16) * ns * 64;
+    if (u >= total)
+        return;
+    const int lane = (int)(u & 63);
+    const int64_t tile = u >> 6;
+    const int r = (int)(tile / ns), s = (int)(tile % ns);
+    const int i = lane & 15, g = lane >> 4;
+    for (int e = 0; e < 8; ++e)
+        out[u * 8 + e] = W[(int64_t)(16 * r + i) * n_in + tile_k(perm, s, g, e)];
+}
+
+// Copy N16 16-byte words global -> LDS with the whole block, 8 loads in
+// flight per thread before their LDS writes (a plain copy loop waits for
+// every load before the next one is issued).
+template <int N16>
+__device__ __forceinline__ void stage_tiles(const uint4 *__restrict__ src,
+                                            uint4 *dst)
+{
+    constexpr int BATCH = N16 / ITHREADS < 8 ? N16 / ITHREADS : 8;
+    static_assert(BATCH > 0 && N16 % (BATCH * ITHREADS) == 0, "whole batches");
+#pragma unroll
+    for (int base = 0; base < N16; base += BATCH * ITHREADS) {
+        uint4 v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u)
+            v[u] = src[base + u * ITHREADS + threadIdx.x];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u)
+            dst[base + u * ITHREADS + threadIdx.x] = v[u];
+    }
+}
+
+// One layer on the wave's column blocks: NS reduction steps of 32, NRB
+// output row blocks of 16.  The bias seeds the accumulator.  Hidden layers
+// write bf16 GELU activations in the next layer's B layout; the output
+// layer keeps FP32 results.  Tiles of row block r + 1 are read from LDS
+// while row block r multiplies.
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4;
+
+// Two GELU activations packed as one bf16 pair, materialised here: the
+// empty asm pins the packing to this point (otherwise the multiplies and
+// conversions sink to the next layer and keep two FP32 values live per
+// activation across the LDS staging).
+__device__ __forceinline__ unsigned gelu_pair(float z0, float z1)
+{
+    bf16x2 p = {f2bf(gelu_t(z0, nullptr)), f2bf(gelu_t(z1, nullptr))};
+    unsigned w = __builtin_bit_cast(unsigned, p);
+    asm volatile("" : "+v"(w));
+    return w;
+}
+
+template <int NS, int NRB, bool HIDDEN>
+__device__ __forceinline__ void infer_layer(const bf16x8 *wt,
+                                            const float *__restrict__ bias,
+                                            const u32x4 (&in)[INC][8],
+                                            u32x4 (&nxt)[INC][8],
+                                            floatx4 (&res)[INC][4], int lane)
+{
+    const int g = lane >> 4;
+    bf16x8 a_cur[NS], a_nxt[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        a_cur[s] = wt[s * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < NRB; ++r) {
+        if (r + 1 < NRB) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                a_nxt[s] = wt[((r + 1) * NS + s) * 64 + lane];
+        }
+        const floatx4 bv = *(const floatx4 *)(bias + 16 * r + 4 * g);
+        floatx4 acc[INC];
+#pragma unroll
+        for (int b = 0; b < INC; ++b)
+            acc[b] = bv;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+#pragma unroll
+            for (int b = 0; b < INC; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a_cur[s], __builtin_bit_cast(bf16x8, in[b][s]), acc[b],
+                    0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < INC; ++b) {
+            if (HIDDEN) {
+                nxt[b][r >> 1][(r & 1) * 2] = gelu_pair(acc[b][0], acc[b][1]);
+                nxt[b][r >> 1][(r & 1) * 2 + 1] = gelu_pair(acc[b][2], acc[b][3]);
+            } else {
+                res[b][r] = acc[b];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            a_cur[s] = a_nxt[s];
+        // one row block at a time: left free, the scheduler interleaves the
+        // epilogues of many row blocks and runs out of registers; the other
+        // wave on the SIMD overlaps this wave's epilogue instead
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// sinh for the decode, FP32 arithmetic with an FP64 result (the decoded
+// rate carries the surrogate's own error, far above FP32 rounding): odd
+// Taylor series below 1/2, exponentials up to 20, and above that 0.5 e^|x|
+// formed as 2^64 * (0.5 e^|x| 2^-64) so the FP32 exponential stays finite up
+// to |x| = 132 (beyond, +-inf, which the gate sends to the exact kernel).
+__device__ __forceinline__ double sinh_decode(float x)
+{
+    const float a = fabsf(x);
+    if (a < 0.5f) {
+        const float x2 = x * x;
+        return (double)(x * (1.0f + x2 * (1.0f / 6.0f + x2 * (1.0f / 120.0f +
+                                                               x2 * (1.0f / 5040.0f)))));
+    }
+    if (a < 20.0f) {
+        const float e = __builtin_amdgcn_exp2f(a * 1.4426950408889634f);
+        return (double)copysignf(0.5f * (e - __builtin_amdgcn_rcpf(e)), x);
+    }
+    const float e = __builtin_amdgcn_exp2f(a * 1.4426950408889634f - 64.0f);
+    return copysign(__builtin_ldexp((double)(0.5f * e), 64), (double)x);
+}
+
+// Per-species constants of the state stages, staged once per block.
+struct InferConsts {
+    float fmu[IKIN], fsd[IKIN];      // feature normalisation
+    float ymu[IKOUT], ysd[IKOUT];    // output standardisation
+    double scale[IKOUT], mw[IKOUT];  // arcsinh scale, molecular weight
+    float elem[8][IKOUT];            // element counts
+};
+
+// Partial gate sums of the second half of a state's species.
+struct GatePart {
+    double mres, mscale, er[8], es[8];
+    int bad;
+};
+
+static_assert(ISTATES * IOUT_LD * 4 + ISTATES * (int)sizeof(GatePart) <= ILDS,
+              "output stage and gate partials fit");
+
+__global__ __launch_bounds__(ITHREADS) void k_mlp_infer(
+    const double *__restrict__ T, const double *__restrict__ P,
+    const double *__restrict__ Y, int64_t n, int64_t ld, int nsp,
+    const __bf16 *__restrict__ wt, const float *__restrict__ bias,
+    const float *__restrict__ mu, const float *__restrict__ sd,
+    const double *__restrict__ scale, const float *__restrict__ ymu,
+    const float *__restrict__ ysd, double *__restrict__ W,
+    const crs_mlp_gate_params gp, const double *__restrict__ elem, int n_elem,
+    const double *__restrict__ mw, int32_t *__restrict__ flags,
+    int32_t *__restrict__ fb_idx, int32_t *__restrict__ fb_count)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char smem[ILDS];
+    __shared__ InferConsts cst;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    // state stages: thread t works on state t % ISTATES, species of parity
+    // t / ISTATES (two threads per state)
+    const int st = threadIdx.x % ISTATES;
+    const int half = threadIdx.x / ISTATES;
+    const int64_t m = (int64_t)blockIdx.x * ISTATES + st;
+    const bool valid = m < n;
+
+    for (int c = threadIdx.x; c < IKIN; c += ITHREADS) {
+        const bool in = c < nsp + 2;
+        cst.fmu[c] = in ? mu[c] : 0.0f;
+        cst.fsd[c] = in ? sd[c] : 1.0f;
+    }
+    for (int k = threadIdx.x; k < IKOUT; k += ITHREADS) {
+        const bool in = k < nsp;
+        cst.ymu[k] = in ? ymu[k] : 0.0f;
+        cst.ysd[k] = in ? ysd[k] : 0.0f;
+        cst.scale[k] = in ? scale[k] : 0.0;
+        cst.mw[k] = in ? mw[k] : 0.0;
+        for (int e = 0; e < 8; ++e)
+            cst.elem[e][k] = (in && e < n_elem) ? (float)elem[e * nsp + k] : 0.0f;
+    }
+    __syncthreads();
+
+    // ---- features of the state, composition checks ---------------------
+    // The state's two threads split its species by parity; each issues all
+    // of its (at most 32) loads before using any, then the halves exchange
+    // their partial mass-fraction sums and composition checks through LDS.
+    // Every element 0..IKIN-1 of the state's row is written exactly once.
+    int bad = 0;
+    double *xsum = (double *)(smem + ISTATES * IFEAT_LD * 2);
+    int *xbad = (int *)(xsum + ITHREADS);
+    {
+        __bf16 *frow = (__bf16 *)smem + (int64_t)st * IFEAT_LD;
+        if (half == 0)
+            for (int c = nsp + 2; c < IKIN; ++c)
+                frow[c] = f2bf(0.0f);
+        double y[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int k = 2 * u + half;
+            y[u] = (valid && k < nsp - 1) ? Y[(int64_t)k * ld + m] : 0.0;
+        }
+        double psum = 0.0;
+        int pbad = 0;
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int k = 2 * u + half;
+            if (k < nsp - 1) {
+                if (!(y[u] >= 0.0 && y[u] <= 1.0))
+                    pbad = 1;
+                psum += y[u];
+                frow[2 + k] = valid ? f2bf((ln_feature(y[u]) - cst.fmu[2 + k]) /
+                                           cst.fsd[2 + k])
+                                    : f2bf(0.0f);
+            }
+        }
+        xsum[threadIdx.x] = psum;
+        xbad[threadIdx.x] = pbad;
+        if (valid) {
+            const double t = T[m], p = P[m];
+            if (!(t >= gp.t_lo && t <= gp.t_hi && p >= gp.p_lo && p <= gp.p_hi))
+                pbad = 1;
+            frow[half] = f2bf((ln_feature(half ? p : t) - cst.fmu[half]) /
+                              cst.fsd[half]);
+        } else {
+            frow[half] = f2bf(0.0f);
+        }
+        __syncthreads();
+        const int other = threadIdx.x ^ ISTATES;
+        const double ssum = psum + xsum[other];
+        bad = pbad | xbad[other];
+        if (!(ssum <= 1.0 + 1e-12))
+            bad = 1;
+        if (((nsp - 1) & 1) == half)
+            frow[1 + nsp] = valid ? f2bf((ln_feature(1.0 - ssum) -
+                                          cst.fmu[1 + nsp]) / cst.fsd[1 + nsp])
+                                  : f2bf(0.0f);
+    }
+    __syncthreads();
+    u32x4 act[INC][8], nxt[INC][8];
+    floatx4 res[INC][4];
+    {
+        const __bf16 *fw =
+            (const __bf16 *)smem + (int64_t)(wave * 16 * INC) * IFEAT_LD;
+#pragma unroll
+        for (int b = 0; b < INC; ++b)
+#pragma unroll
+            for (int s = 0; s < IKIN / 32; ++s)
+                act[b][s] = *(const u32x4 *)(fw + (16 * b + j) * IFEAT_LD +
+                                             32 * s + 8 * g);
+    }
+
+    // ---- layers --------------------------------------------------------
+    const bf16x8 *lt = (const bf16x8 *)smem;
+    const uint4 *src = (const uint4 *)wt;
+    constexpr int n0 = IH * IKIN * 2 / 16, n1 = IH * IH * 2 / 16;
+    constexpr int n3 = IKOUT * IH * 2 / 16;
+    __syncthreads();
+    stage_tiles<n0>(src, (uint4 *)smem);
+    __syncthreads();
+    infer_layer<IKIN / 32, IH / 16, true>(lt, bias, act, nxt, res, lane);
+    src += n0;
+    __syncthreads();
+    stage_tiles<n1>(src, (uint4 *)smem);
+    __syncthreads();
+    infer_layer<IH / 32, IH / 16, true>(lt, bias + IH, nxt, act, res, lane);
+    src += n1;
+    __syncthreads();
+    stage_tiles<n1>(src, (uint4 *)smem);
+    __syncthreads();
+    infer_layer<IH / 32, IH / 16, true>(lt, bias + 2 * IH, act, nxt, res, lane);
+    src += n1;
+    __syncthreads();
+    stage_tiles<n3>(src, (uint4 *)smem);
+    __syncthreads();
+    infer_layer<IH / 32, IKOUT / 16, false>(lt, bias + 3 * IH, nxt, act, res,
+                                            lane);
+    __syncthreads();
+
+    // ---- outputs to per-state rows -------------------------------------
+    {
+        float *ow = (float *)smem + (int64_t)(wave * 16 * INC) * IOUT_LD;
+#pragma unroll
+        for (int b = 0; b < INC; ++b)
+#pragma unroll
+            for (int r = 0; r < IKOUT / 16; ++r)
+                *(floatx4 *)(ow + (16 * b + j) * IOUT_LD + 16 * r + 4 * g) =
+                    res[b][r];
+    }
+    __syncthreads();
+    GatePart *parts = (GatePart *)(smem + ISTATES * IOUT_LD * 4);
+    GatePart gpart = {};
+    if (valid) {
+        // decode this thread's species (parity = half), partial balances
+        const float *orow = (const float *)smem + (int64_t)st * IOUT_LD;
+#pragma unroll 2
+        for (int k = half; k < nsp; k += 2) {
+            const float ya = cst.ymu[k] + cst.ysd[k] * orow[k];
+            const double w = sinh_decode(ya) * cst.scale[k];
+            W[(int64_t)k * ld + m] = w;
+            if (!isfinite(w))
+                gpart.bad = 1;
+            gpart.mres += cst.mw[k] * w;
+            gpart.mscale += cst.mw[k] * fabs(w);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const double a = (double)cst.elem[e][k];
+                gpart.er[e] += a * w;
+                gpart.es[e] += a * fabs(w);
+            }
+        }
+        if (half == 1)
+            parts[st] = gpart;
+    }
+    __syncthreads();
+    if (half == 0) {
+        if (valid) {
+            const GatePart &o = parts[st];
+            bad |= gpart.bad | o.bad;
+            const double mres = gpart.mres + o.mres;
+            const double mscale = gpart.mscale + o.mscale;
+            if (mscale > 0.0 && fabs(mres) > gp.rmax * mscale)
+                bad = 1;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const double r = gpart.er[e] + o.er[e];
+                const double sc = gpart.es[e] + o.es[e];
+                if (sc > 0.0 && fabs(r) > gp.rmax * sc)
+                    bad = 1;
+            }
+            flags[m] = bad;
+        }
+        // ---- fallback list: one atomic per wave ------------------------
+        const uint64_t ballot = __ballot(valid && bad);
+        if (ballot) {
+            const int leader = __ffsll((unsigned long long)ballot) - 1;
+            int base = 0;
+            if (lane == leader)
+                base = atomicAdd(fb_count, (int)__popcll(ballot));
+            base = __shfl(base, leader);
+            if (valid && bad)
+                fb_idx[base + __popcll(ballot & ((1ull << lane) - 1ull))] =
+                    (int32_t)m;
+        }
     }
 }
 
@@ -555,6 +961,56 @@ int crs_mlp_gate(const double *T, const double *P, const double *Y,
         return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     k_gate<<<crs::grid_for(n), crs::kBlock, 0, (hipStream_t)stream>>>(
         T, P, Y, W, n, ld, nsp, gp, elem, n_elem, mw, flags, fb_idx, fb_count);
+    return crs::launched(fn);
+}
+
+int crs_mlp_pack_tiles(const void *W, int n_out, int n_in, int perm,
+                       void *out, void *stream)
+{
+    const char *fn = "crs_mlp_pack_tiles";
+    if (n_out <= 0 || n_in <= 0 || n_out % 16 || n_in % 32)
+        return crs::fail(CRS_ERR_ARG, "%s: bad shape %d x %d", fn, n_out, n_in);
+    if (!W || !out)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
+    const int64_t total = (int64_t)n_out / 16 * (n_in / 32) * 64;
+    k_pack_tiles<<<crs::grid_for(total), crs::kBlock, 0, (hipStream_t)stream>>>(
+        (const __bf16 *)W, n_out, n_in, perm, (__bf16 *)out);
+    return crs::launched(fn);
+}
+
+int crs_mlp_infer(const double *T, const double *P, const double *Y,
+                  int64_t n, int64_t ld, int nsp, int kin, int hidden,
+                  int layers, int kout, const void *tiles, const float *bias,
+                  const float *mu, const float *sd, const double *scale,
+                  const float *ymu, const float *ysd, double *W,
+                  crs_mlp_gate_params gp, const double *elem, int n_elem,
+                  const double *mw, int32_t *flags, int32_t *fb_idx,
+                  int32_t *fb_count, void *stream)
+{
+    const char *fn = "crs_mlp_infer";
+    if (kin != IKIN || hidden != IH || layers != 3 || kout != IKOUT)
+        return crs::fail(CRS_ERR_UNSUPPORTED,
+                         "%s: fused kernel is built for %d -> 3 x %d -> %d "
+                         "(got %d -> %d x %d -> %d)", fn, IKIN, IH, IKOUT, kin,
+                         layers, hidden, kout);
+    if (n < 0 || nsp < 2 || nsp + 2 > IKIN || nsp > IKOUT || ld < n ||
+        n_elem < 0 || n_elem > 8)
+        return crs::fail(CRS_ERR_ARG, "%s: bad sizes n=%lld ld=%lld nsp=%d "
+                         "n_elem=%d", fn, (long long)n, (long long)ld, nsp,
+                         n_elem);
+    if (n == 0)
+        return CRS_OK;
+    if (!T || !P || !Y || !tiles || !bias || !mu || !sd || !scale || !ymu ||
+        !ysd || !W || !mw || !flags || !fb_idx || !fb_count ||
+        (n_elem > 0 && !elem))
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
+    if (((uintptr_t)tiles & 15) || ((uintptr_t)bias & 15))
+        return crs::fail(CRS_ERR_ARG, "%s: tiles and bias need 16-byte "
+                         "alignment", fn);
+    const dim3 grid((unsigned)((n + ISTATES - 1) / ISTATES));
+    k_mlp_infer<<<grid, ITHREADS, 0, (hipStream_t)stream>>>(
+        T, P, Y, n, ld, nsp, (const __bf16 *)tiles, bias, mu, sd, scale, ymu,
+        ysd, W, gp, elem, n_elem, mw, flags, fb_idx, fb_count);
     return crs::launched(fn);
 }
 

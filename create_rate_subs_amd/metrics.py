@@ -15,11 +15,16 @@ well-conditioned entries) this is the plain relative error; below, errors
 are measured relative to tau times the magnitude of the terms being summed.
 An entry with w_hat == w scores 0 (this covers species in no reaction,
 where w = S = 0).
+
+For the learned surrogate a per-state normwise error is also reported,
+max_k |w_hat_k - w_k| / max_k |w_k|: the error relative to the state's
+largest rate, the usual figure for vector-valued surrogates.
 """
 
 import numpy as np
 
-__all__ = ['nu_matrix', 'gross_scale', 'rel_err', 'summarize']
+__all__ = ['nu_matrix', 'gross_scale', 'rel_err', 'normwise_err',
+           'summarize']
 
 
 def _is_torch(x):
@@ -94,3 +99,25 @@ def summarize(err):
     e = np.asarray(err).ravel()
     return dict(max=float(e.max()), median=float(np.median(e)),
                 p999=float(np.quantile(e, 0.999)))
+
+
+def normwise_err(w_hat, w):
+    """Per-state max_k |w_hat_k - w_k| / max_k |w_k| over rows k ([NSP][n]).
+
+    0 where both are identical (including all-zero states); inf where the
+    exact rates are all zero but the estimate is not.
+    """
+    if _is_torch(w):
+        import torch
+        d = (w_hat - w).abs().amax(dim=0)
+        m = w.abs().amax(dim=0)
+        out = d / m
+        out = torch.where(d == 0, torch.zeros_like(out), out)
+        return torch.where(torch.isnan(out), torch.full_like(out, float('inf')),
+                           out)
+    d = np.abs(w_hat - w).max(axis=0)
+    m = np.abs(w).max(axis=0)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        out = d / m
+    out = np.where(d == 0, 0.0, out)
+    return np.where(np.isnan(out), np.inf, out)

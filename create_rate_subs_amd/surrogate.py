@@ -78,6 +78,10 @@ class MlpLib(object):
         'crs_mlp_to_bf16': [_P, _P, _I64, _P],
         'crs_mlp_gate': [_P, _P, _P, _P, _I64, _I64, _I, GateParams, _P, _I,
                          _P, _P, _P, _P, _P],
+        'crs_mlp_pack_tiles': [_P, _I, _I, _I, _P, _P],
+        'crs_mlp_infer': [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _I, _P, _P,
+                          _P, _P, _P, _P, _P, _P, GateParams, _P, _I, _P, _P,
+                          _P, _P, _P],
     }
 
     def __init__(self, path=MLP_LIB):
@@ -328,8 +332,52 @@ class Surrogate(object):
         return history
 
     # -- inference ----------------------------------------------------------
-    def predict(self, T, P, Y, W):
+    @property
+    def fused_ok(self):
+        """The fused kernel covers this network's shape."""
+        return (self.kin, self.hid, len(self.shapes) - 1, self.kout) == \
+            (64, 256, 3, 64)
+
+    def pack(self):
+        """Weights in the fused kernel's MFMA tile order, biases in one
+        array; redone whenever the parameters changed since the last pack."""
+        if getattr(self, '_packed_step', None) == self.step and \
+                getattr(self, '_packed_id', None) == id(self.pb):
+            return self._packed
+        tiles = self._buf('tiles', (sum(a * b for a, b in self.shapes),),
+                          torch.bfloat16)
+        off = 0
+        for li, (n_out, n_in) in enumerate(self.shapes):
+            cnt = n_out * n_in
+            self.mlp('crs_mlp_pack_tiles', _p(self.w(li)), n_out, n_in,
+                     1 if li > 0 else 0, _p(tiles[off:off + cnt]), _s())
+            off += cnt
+        bias = torch.cat([self.b(li) for li in range(len(self.shapes))])
+        self._packed = (tiles, bias.contiguous())
+        self._packed_step, self._packed_id = self.step, id(self.pb)
+        return self._packed
+
+    def infer(self, T, P, Y, W):
+        """Fused surrogate + gate over every state: W [NSP][ld] gets the
+        surrogate rates; returns (flags, fallback indices, count tensor)."""
+        n = T.numel()
+        tiles, bias = self.pack()
+        flags = self._buf('flags', (n,), torch.int32)
+        idx = self._buf('fb_idx', (n,), torch.int32)
+        cnt = torch.zeros(1, dtype=torch.int32, device='cuda')
+        self.mlp('crs_mlp_infer', _p(T), _p(P), _p(Y), n, Y.stride(0),
+                 self.nsp, self.kin, self.hid, len(self.shapes) - 1,
+                 self.kout, _p(tiles), _p(bias), _p(self.mu), _p(self.sd),
+                 _p(self.scale), _p(self.ymu), _p(self.ysd), _p(W), self.gate,
+                 _p(self.elem), self.elem.shape[0], _p(self.mw), _p(flags),
+                 _p(idx), _p(cnt), _s())
+        return flags, idx, cnt
+
+    def predict(self, T, P, Y, W, fused=True):
         """Surrogate wdot for every state into W [NSP][ld] (FP64)."""
+        if fused and self.fused_ok:
+            self.infer(T, P, Y, W)
+            return W
         X = self.features(T, P, Y)
         Yh = self.forward(X)
         n = T.numel()
@@ -348,10 +396,13 @@ class Surrogate(object):
                  _p(self.mw), _p(flags), _p(idx), _p(cnt), _s())
         return flags, idx, cnt
 
-    def evaluate(self, T, P, Y, W, mode='fp64'):
+    def evaluate(self, T, P, Y, W, mode='fp64', fused=True):
         """Gated surrogate: predict, gate, exact fallback.  Returns m."""
-        self.predict(T, P, Y, W)
-        flags, idx, cnt = self.gate_states(T, P, Y, W)
+        if fused and self.fused_ok:
+            flags, idx, cnt = self.infer(T, P, Y, W)
+        else:
+            self.predict(T, P, Y, W, fused=False)
+            flags, idx, cnt = self.gate_states(T, P, Y, W)
         m = int(cnt.item())
         if m:
             self.mech.eval_wdot_subset(T, P, Y, W, idx[:m], mode)
@@ -392,18 +443,23 @@ class Surrogate(object):
         return r.max(dim=0).values
 
     def calibrate_gate(self, T, P, Y, W_exact, S, target, tau=5e-2,
-                       curve=(1e-2, 3e-2, 1e-1, 3e-1, 1.0)):
+                       curve=(1e-2, 3e-2, 1e-1, 3e-1, 1.0),
+                       metric='conditioned'):
         """Largest rmax whose admitted states all meet ``target`` error.
 
         States are ranked by their balance residual; the admitted set is the
-        longest low-residual prefix whose worst conditioned error (max over
-        species, create_rate_subs_amd.metrics) is within the target.  Sets
+        longest low-residual prefix whose worst error is within the target:
+        the conditioned error (max over species, create_rate_subs_amd.metrics)
+        or, with metric='normwise', the per-state normwise error.  Sets
         gate.rmax and returns (rmax, admitted fraction, curve) where curve
         lists the admitted fraction for each error target in ``curve``.
         """
         W = torch.empty_like(W_exact)
         self.predict(T, P, Y, W)
-        e = metrics.rel_err(W, W_exact, S, tau).max(dim=0).values
+        if metric == 'normwise':
+            e = metrics.normwise_err(W, W_exact)
+        else:
+            e = metrics.rel_err(W, W_exact, S, tau).max(dim=0).values
         e = torch.where(torch.isfinite(e), e, torch.full_like(e, 1e300))
         r = self.residuals(W)
         r = torch.where(torch.isfinite(r), r, torch.full_like(r, 1e300))
@@ -462,3 +518,4 @@ class Surrogate(object):
                                g['rmax'])
         self.mlp('crs_mlp_to_bf16', _p(self.p), _p(self.pb), self.nparam,
                  _s())
+        self._packed_step = None

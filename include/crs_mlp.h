@@ -96,6 +96,30 @@ int crs_mlp_gate(const double *T, const double *P, const double *Y,
                  const double *mw, int32_t *flags, int32_t *fb_idx,
                  int32_t *fb_count, void *stream);
 
+/* Rearrange one bf16 weight matrix W [n_out][n_in] (row major) into the
+ * fused inference kernel's MFMA tile order: tiles of 16 rows x 32 inputs,
+ * row block major, each as 64 lanes x 8 bf16.  perm != 0 for every layer
+ * fed by a hidden layer (its input index follows the MFMA output layout).
+ * n_out % 16 == 0, n_in % 32 == 0. */
+int crs_mlp_pack_tiles(const void *W, int n_out, int n_in, int perm,
+                       void *out, void *stream);
+
+/* Fused surrogate evaluation of n states: features, the 64 -> 3 x 256 (GELU)
+ * -> 64 network from the packed tiles (layers back to back) and biases
+ * (fp32, 256 + 256 + 256 + 64, 16-byte aligned), decode
+ * wdot_k = scale_k sinh(ymu_k + ysd_k y_k) into W [nsp][ld], and the gate of
+ * crs_mlp_gate (flags, fallback list, count added to *fb_count).  Only the
+ * 64/256/3/64 shape is built (CRS_ERR_UNSUPPORTED otherwise); nsp <= 62,
+ * n_elem <= 8. */
+int crs_mlp_infer(const double *T, const double *P, const double *Y,
+                  int64_t n, int64_t ld, int nsp, int kin, int hidden,
+                  int layers, int kout, const void *tiles, const float *bias,
+                  const float *mu, const float *sd, const double *scale,
+                  const float *ymu, const float *ysd, double *W,
+                  crs_mlp_gate_params gp, const double *elem, int n_elem,
+                  const double *mw, int32_t *flags, int32_t *fb_idx,
+                  int32_t *fb_count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

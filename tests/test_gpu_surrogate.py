@@ -121,6 +121,68 @@ def test_gate_fallback_is_exact_and_catches_invalid_states():
     assert audit['states'] == 250
 
 
+def _fit_random(name, n, seed):
+    """Network with normalisation fitted to n labelled states, no training."""
+    net = _net(name, seed=seed)
+    Td, Pd, Yd, W = _labelled(name, n, seed)
+    net.fit_normalisation(Td, Pd, Yd, W)
+    net.p.add_(torch.randn_like(net.p) * 0.02)
+    lib = load_mlp()
+    lib('crs_mlp_to_bf16', _p(net.p), _p(net.pb), net.nparam, _s())
+    net.step += 1
+    return net, Td, Pd, Yd
+
+
+@pytest.mark.parametrize('n', [5, 1000, 4096])
+def test_fused_inference_matches_layered(n):
+    """One-kernel inference (activations in registers, permuted tiles)
+    against the layer-by-layer GEMM path, in the network's output space."""
+    net, Td, Pd, Yd = _fit_random('mech13', n, 60 + n)
+    Wf = torch.empty((net.nsp, n), dtype=torch.float64, device='cuda')
+    Wl = torch.empty_like(Wf)
+    net.predict(Td, Pd, Yd, Wf, fused=True)
+    net.predict(Td, Pd, Yd, Wl, fused=False)
+    s = net.scale[:, None]
+    yf = (torch.asinh(Wf / s) - net.ymu[:, None].double()) / net.ysd[:, None].double()
+    yl = (torch.asinh(Wl / s) - net.ymu[:, None].double()) / net.ysd[:, None].double()
+    # the fused decode covers |arcsinh| <= 132 (FP32 exponential); beyond it
+    # returns inf, which the gate sends to the exact kernel
+    big = torch.asinh(Wl / s).abs() > 130.0
+    ok = torch.isfinite(yf) & torch.isfinite(yl)
+    assert bool((ok | big).all())
+    d = (yf - yl).abs()[ok]
+    print('n', n, 'max', d.max().item(), 'mean', d.mean().item(),
+          'beyond range', int(big.sum().item()))
+    # bf16 activations may round differently (summation order differs)
+    assert d.max().item() < 3e-2 and d.mean().item() < 2e-3
+
+
+def test_fused_gate_flags_match_gate_kernel():
+    name = 'mech13'
+    net, Td, Pd, Yd = _fit_random(name, 3000, 77)
+    Td[10] = 1.0e4                    # outside the training box
+    Yd[2, 20] = -0.5                  # invalid composition
+    Yd[4, 30] = float('nan')
+    for rmax in (1e300, 1e-1, 1e-3):
+        net.gate.rmax = rmax
+        W1 = torch.empty((net.nsp, 3000), dtype=torch.float64, device='cuda')
+        f1, i1, c1 = net.infer(Td, Pd, Yd, W1)
+        f1 = f1.clone()
+        i1 = i1[:int(c1.item())].clone()
+        f2, i2, c2 = net.gate_states(Td, Pd, Yd, W1)
+        if rmax > 1e100:
+            # box, composition and finiteness only: identical decisions
+            assert torch.equal(f1, f2)
+            assert torch.equal(i1.sort().values,
+                               i2[:int(c2.item())].sort().values)
+        else:
+            # balance sums are added in another order: the threshold may
+            # fall differently for a state right at it
+            assert int((f1 != f2).sum().item()) <= 3
+        assert f1[10] and f1[20] and f1[30]
+        print('rmax', rmax, 'fallback', int(c1.item()))
+
+
 def test_checkpoint_roundtrip(tmp_path):
     net = _net(seed=17)
     Td, Pd, Yd, W = _labelled('mech13', 1 << 13, 53)
