@@ -605,7 +605,7 @@ def _fused_f64_lines(ir, K, opts):
         lines.append('}')
         if opts.get('early_store'):
             for k, _nu in x.spec:
-                if last[k] == x.index:
+                if last.get(k) == x.index:
                     lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
@@ -1055,6 +1055,73 @@ def _species_updates(x, seen):
     return out
 
 
+_MF_CHUNK = 32   # reactions per contraction step (8 MFMA K-steps of 4)
+
+
+def _mfma_nu_table(ir, rxns):
+    """nu^T in v_mfma_f32_16x16x4f32 A-operand order.
+
+    For chunk c of _MF_CHUNK reactions (in emission order), species tile st
+    (16 species), K-step ks (4 reactions) and lane l, the value is
+    nu[16 st + l % 16][reaction 32 c + 4 ks + l / 16] (0 outside).
+    """
+    nst = (ir.nsp + 15) // 16
+    nch = (len(rxns) + _MF_CHUNK - 1) // _MF_CHUNK
+    vals = []
+    for c in range(nch):
+        for st in range(nst):
+            for ks in range(_MF_CHUNK // 4):
+                for lane in range(64):
+                    k = 16 * st + lane % 16
+                    j = _MF_CHUNK * c + 4 * ks + lane // 16
+                    v = 0.0
+                    if k < ir.nsp and j < len(rxns):
+                        for kk, nu in rxns[j].spec:
+                            if kk == k:
+                                v = float(nu)
+                    vals.append(v)
+    return nst, nch, vals
+
+
+def _mfma_chunk_end(pos, nst):
+    """Contraction after reaction position ``pos`` closes a chunk (the
+    rest of a partial last chunk is zero-filled first)."""
+    c = pos // _MF_CHUNK
+    out = []
+    for slot in range(pos % _MF_CHUNK + 1, _MF_CHUNK):
+        out.append('sQ[threadIdx.x >> 6][{}][threadIdx.x & 63] = 0.0f;'.format(
+            slot))
+    out.append('__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
+    return out + _mfma_contract_lines(c, nst)
+
+
+def _mfma_contract_lines(c, nst):
+    """D[st][ct] += nu^T(chunk c) x Q(chunk) for the wave's 4 state tiles."""
+    out = ['{  // contraction of chunk %d on MFMA' % c,
+           '    const float *qw = &sQ[threadIdx.x >> 6][0][0];',
+           '    const int lq = threadIdx.x & 63;']
+    out.append('    #pragma unroll')
+    out.append('    for (int ks = 0; ks < {}; ++ks) {{'.format(_MF_CHUNK // 4))
+    out.append('        float bq[4];')
+    out.append('        #pragma unroll')
+    out.append('        for (int ct = 0; ct < 4; ++ct)')
+    out.append('            bq[ct] = qw[(4 * ks + (lq >> 4)) * 64 + 16 * ct + '
+               '(lq & 15)];')
+    out.append('        #pragma unroll')
+    out.append('        for (int st = 0; st < {}; ++st) {{'.format(nst))
+    out.append('            const float a = crs_nuA[(({c} * {nst} + st) * {ks} + ks)'
+               ' * 64 + lq];'.format(c=c, nst=nst, ks=_MF_CHUNK // 4))
+    out.append('            #pragma unroll')
+    out.append('            for (int ct = 0; ct < 4; ++ct)')
+    out.append('                Dq[st][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32('
+               'a, bq[ct], Dq[st][ct], 0, 0, 0);')
+    out.append('        }')
+    out.append('    }')
+    out.append('    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
+    out.append('}')
+    return out
+
+
 def _fused_fast_lines(ir, K, opts, mixed=False):
     """Per-state body of the fused FP64 kernel, arranged for the GPU.
 
@@ -1135,11 +1202,23 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
     rxns = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
     last = _last_touch(ir, rxns)
     seen = [False] * nsp
-    lines.append('double w[{}];'.format(nsp))
-    for k in range(nsp):
-        if k not in last:
-            lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
-    for x in rxns:
+    mfma = mixed and opts.get('contraction') == 'mfma'
+    if mfma:
+        nst = (nsp + 15) // 16
+        lines.append('__shared__ float sQ[crs::kBlock / 64][{}][64];'.format(
+            _MF_CHUNK))
+        lines.append('floatx4 Dq[{}][4];'.format(nst))
+        lines.append('#pragma unroll')
+        lines.append('for (int a = 0; a < {}; ++a)'.format(nst))
+        lines.append('    for (int b = 0; b < 4; ++b)')
+        lines.append('        Dq[a][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};')
+        last = {}
+    else:
+        lines.append('double w[{}];'.format(nsp))
+        for k in range(nsp):
+            if k not in last:
+                lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
+    for pos, x in enumerate(rxns):
         inner, xf = _fast_arr(x.arr, K, 'kf', expfn)
         if hg and _hg_ok(x):
             pm = ''
@@ -1148,12 +1227,19 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                           _fast_pm_lines(x, K, xf))
                 pm = ' * pm'
             inner += _hg_net_rate(x, gref, pm)
-            inner += _species_updates(x, seen)
+            if mfma:
+                inner.append('sQ[threadIdx.x >> 6][{}][threadIdx.x & 63] = '
+                             '(float)q;'.format(pos % _MF_CHUNK))
+            else:
+                inner += _species_updates(x, seen)
             lines.append('{  // rxn %d' % x.index)
             lines += ['    ' + l for l in inner]
             lines.append('}')
+            if mfma and (pos % _MF_CHUNK == _MF_CHUNK - 1 or
+                         pos == len(rxns) - 1):
+                lines += _mfma_chunk_end(pos, nst)
             for k, _nu in x.spec:
-                if last[k] == x.index:
+                if last.get(k) == x.index:
                     lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
             continue
         inner.append('const double f = {}kf;'.format(_conc_prod(x.reac)))
@@ -1202,15 +1288,41 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                       _fast_pm_lines(x, K, xf))
             q = q + ' * pm'
         inner.append('const double q = {};'.format(q))
-        inner += _species_updates(x, seen)
+        if mfma:
+            inner.append('sQ[threadIdx.x >> 6][{}][threadIdx.x & 63] = '
+                         '(float)q;'.format(pos % _MF_CHUNK))
+        else:
+            inner += _species_updates(x, seen)
         lines.append('{  // rxn %d' % x.index)
         lines += ['    ' + l for l in inner]
         lines.append('}')
+        if mfma and (pos % _MF_CHUNK == _MF_CHUNK - 1 or pos == len(rxns) - 1):
+            lines += _mfma_chunk_end(pos, nst)
         for k, _nu in x.spec:
-            if last[k] == x.index:
+            if last.get(k) == x.index:
                 lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
+    if mfma:
+        # D[st][ct]: lane l holds species 16 st + 4 (l / 16) + t of the
+        # state in lane 16 ct + l % 16 of this wave
+        lines.append('{')
+        lines.append('    const int lq = threadIdx.x & 63;')
+        lines.append('    #pragma unroll')
+        lines.append('    for (int ct = 0; ct < 4; ++ct) {')
+        lines.append('        const int src = 16 * ct + (lq & 15);')
+        lines.append('        const long long iq = __shfl((long long)i, src);')
+        lines.append('        const int ok = __shfl((int)live, src);')
+        lines.append('        #pragma unroll')
+        lines.append('        for (int st = 0; st < {}; ++st)'.format(nst))
+        lines.append('            #pragma unroll')
+        lines.append('            for (int t = 0; t < 4; ++t) {')
+        lines.append('                const int k = 16 * st + 4 * (lq >> 4) + t;')
+        lines.append('                if (ok && k < {})'.format(nsp))
+        lines.append('                    W[k * ld + iq] = (double)Dq[st][ct][t];')
+        lines.append('            }')
+        lines.append('    }')
+        lines.append('}')
     if hg:
         # concentrations after the H_k lines: C_k = G_k * H_k
         def c_of(mo):
@@ -1363,7 +1475,37 @@ def emit_hip_source(ir, opts=None):
         '    wdot_f64_state(K, T_, P_, Y, W, ld, i);\n}\n\n')
 
     # ---- fused mixed-precision per-state body -------------------------
-    if opts.get('mixed', True):
+    if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
+        rx = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
+        nst, nch, vals = _mfma_nu_table(ir, rx)
+        body.append('typedef __attribute__((__vector_size__(4 * sizeof(float)))) '
+                    'float floatx4;\n')
+        body.append('__device__ const float crs_nuA[{}] = {{\n'.format(len(vals)) +
+                    ''.join('{}f,{}'.format(repr(v), '\n' if q % 16 == 15 else ' ')
+                            for q, v in enumerate(vals)) + '};\n\n')
+        body.append(
+            '__device__ __forceinline__ void wdot_mixed_state(\n'
+            '    const double *__restrict__ K,\n'
+            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Y, double *__restrict__ W,\n'
+            '    int64_t ld, int64_t i, bool live)\n{\n' +
+            _indent(_fused_fast_lines(ir, K, opts, mixed=True), 4) +
+            '}\n\n')
+        body.append(
+            '__global__ __launch_bounds__(crs::kBlock) void k_wdot_mixed(\n'
+            '    const double *__restrict__ K,\n'
+            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Y, double *__restrict__ W,\n'
+            '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
+            '    // whole waves beyond m leave; within a wave every lane runs (the\n'
+            '    // contraction exchanges data across lanes), dead lanes on state 0\n'
+            '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
+            '    if ((t & ~(int64_t)63) >= m) return;\n'
+            '    int64_t i = (t < m) ? (idx ? (int64_t)idx[t] : t) : 0;\n'
+            '    const bool live = t < m && i >= 0 && i < ld;\n'
+            '    if (!live) i = 0;\n'
+            '    wdot_mixed_state(K, T_, P_, Y, W, ld, i, live);\n}\n\n')
+    elif opts.get('mixed', True):
         body.append(
             '__device__ __forceinline__ void wdot_mixed_state(\n'
             '    const double *__restrict__ K,\n'

@@ -1,0 +1,109 @@
+"""Measure emitter variants of the fused wdot kernels.
+
+Every layout and arithmetic choice of the generated kernels (where the
+per-species arrays live, the Kc form, the reaction order, the nu^T q
+contraction) is an emitter option; the defaults are the measured winners.
+This module builds a variant next to the default library and times both on
+the same states, scoring each against the default FP64 kernel with the
+conditioned error of create_rate_subs_amd.metrics:
+
+    # on a machine with hipcc (no GPU needed): emit and compile
+    python -m create_rate_subs_amd.tune build mech53 mfma '{"contraction": "mfma"}'
+    # on the GPU: time the default and the named variants, 2^20 states
+    python -m create_rate_subs_amd.tune run mech53 mfma
+
+Variants live under _build/variants/<mechanism>/<name>/ with the options
+they were built from (opts.json).  One JSON line per (variant, mode).
+"""
+
+import json
+import os
+import sys
+
+from . import build, hip_emitter, workloads
+
+__all__ = ['variant_dir', 'build_variant', 'run']
+
+TAU = 5e-2
+
+
+def variant_dir(mech, name):
+    return os.path.join(workloads.BUILD_ROOT, 'variants', mech, name)
+
+
+def build_variant(mech, name, opts):
+    """Emit and compile mechanism ``mech`` with emitter options ``opts``."""
+    from .create_rate_subs import load_mechanism
+    mpath, tpath = workloads.mechanism_files(mech)
+    _e, specs, reacs, fwd = load_mechanism(mpath, tpath)
+    out = variant_dir(mech, name)
+    src = hip_emitter.write_hip(out, specs, reacs, name=mech, spec_order=fwd,
+                                opts=opts)
+    lib = build.compile_library(src, out)
+    with open(os.path.join(out, 'opts.json'), 'w') as fh:
+        json.dump(dict(opts=opts, resolved=hip_emitter.resolve_opts(
+            len(specs), opts)), fh, indent=1)
+    return lib
+
+
+def run(mech, names, n=1 << 20, steps=10, seed=2001):
+    """Time the default library and each named variant; print JSON lines."""
+    import torch
+    from . import metrics, runtime, states
+    from .create_rate_subs import load_mechanism
+    mpath, tpath = workloads.mechanism_files(mech)
+    _e, specs, reacs, _f = load_mechanism(mpath, tpath)
+    T, P, Y = states.random_states(len(specs), n, seed)
+    Td, Pd, Yd = (torch.from_numpy(a).cuda() for a in (T, P, Y))
+    base = runtime.load(build.lib_path(mech))
+    ref = base.eval_wdot(Td, Pd, Yd, 'fp64')
+    C = base.eval_conc(Td, Pd, Yd)['C']
+    fwd, rev = base.eval_rxn_rates(Td, Pd, C)
+    pm = base.get_rxn_pres_mod(Td, Pd, C)
+    S = metrics.gross_scale(specs, reacs, fwd, rev, pm)
+    del C, fwd, rev, pm
+    libs = [('default', base)]
+    for name in names:
+        path = os.path.join(variant_dir(mech, name),
+                            'libcrs_{}.so'.format(mech))
+        libs.append((name, runtime.MechLib(path)))
+    out = torch.empty_like(Yd)
+    for name, lib in libs:
+        for mode in ('fp64', 'mixed'):
+            if mode == 'mixed' and not lib.modes & 2:
+                continue
+            for _ in range(2):
+                lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+            torch.cuda.synchronize()
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(steps):
+                lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+            t1.record()
+            torch.cuda.synchronize()
+            ms = t0.elapsed_time(t1) / steps
+            finite = torch.isfinite(out)
+            e = metrics.rel_err(out, ref, S, TAU)
+            e = torch.where(finite, e, torch.zeros_like(e))
+            print(json.dumps(dict(
+                mechanism=mech, variant=name, mode=mode, states=n, ms=ms,
+                states_per_s=n / (ms * 1e-3),
+                nonfinite_fraction=1.0 - finite.double().mean().item(),
+                max_err_vs_default_fp64=e.max().item(), tau=TAU)), flush=True)
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) >= 4 and argv[0] == 'build':
+        print(build_variant(argv[1], argv[2], json.loads(argv[3])))
+    elif len(argv) >= 2 and argv[0] == 'run':
+        run(argv[1], argv[2:])
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())
