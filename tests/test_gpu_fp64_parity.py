@@ -107,6 +107,45 @@ def test_wdot_subset(name):
     assert np.isnan(Wh[:, rest]).all()
 
 
+@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
+def test_wdot_ragged_leading_dimension(mode):
+    """Rows with a stride larger than the state count (views into wider
+    buffers): the same rates as the compact layout, padding untouched."""
+    import torch
+    name = 'mech53'
+    n, ld = 1000, 1037
+    T, P, Y = _states(name, n, 16)
+    lib = hip_lib(name)
+    Td, Pd, Yd = to_dev(T, P, Y)
+    compact = to_host(lib.eval_wdot(Td, Pd, Yd, mode))
+    Ybig = torch.full((lib.nsp, ld), float('nan'), dtype=torch.float64,
+                      device='cuda')
+    Ybig[:, :n] = Yd
+    Wbig = torch.full_like(Ybig, 7.0)
+    lib.eval_wdot(Td, Pd, Ybig[:, :n], mode, out=Wbig[:, :n])
+    Wh = to_host(Wbig)
+    np.testing.assert_array_equal(Wh[:, :n], compact)
+    assert (Wh[:, n:] == 7.0).all()
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
+def test_wdot_subset_matches_full_batch(mode):
+    import torch
+    T, P, Y = _states('mech111', 3000, 17)
+    lib = hip_lib('mech111')
+    Td, Pd, Yd = to_dev(T, P, Y)
+    full = to_host(lib.eval_wdot(Td, Pd, Yd, mode))
+    rng = np.random.default_rng(3)
+    sel = np.unique(rng.integers(0, 3000, 700))
+    idx = torch.tensor(rng.permutation(sel), dtype=torch.int32, device='cuda')
+    W = torch.full_like(Yd, np.nan)
+    lib.eval_wdot_subset(Td, Pd, Yd, W, idx, mode)
+    Wh = to_host(W)
+    np.testing.assert_array_equal(Wh[:, sel], full[:, sel])
+    rest = np.setdiff1d(np.arange(3000), sel)
+    assert np.isnan(Wh[:, rest]).all()
+
+
 def test_wdot_empty_and_bad_args():
     import ctypes
     import torch

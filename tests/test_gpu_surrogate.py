@@ -183,6 +183,56 @@ def test_fused_gate_flags_match_gate_kernel():
         print('rmax', rmax, 'fallback', int(c1.item()))
 
 
+def test_trained_network_gain():
+    """What training adds, measured on held-out states.
+
+    Against the same network untrained, the arcsinh-space error must drop
+    at least tenfold and the per-state normwise error must drop too.  The
+    gated evaluation then keeps every state it does not trust exact
+    (bit-identical to the FP64 kernel); the admitted fraction at a
+    normwise target of 0.3 and both paths' times are printed (the honest
+    gain over exact evaluation depends on that fraction).
+    """
+    from create_rate_subs_amd import metrics
+    name = 'mech13'
+    Td, Pd, Yd, W = _labelled(name, 1 << 16, 70)
+    Tv, Pv, Yv, Wv = _labelled(name, 1 << 13, 71)
+    untrained = _net(name, seed=21)
+    untrained.fit_normalisation(Td, Pd, Yd, W)
+    trained = _net(name, seed=21)
+    trained.train(Td, Pd, Yd, W, epochs=12, batch=4096, lr=2e-3)
+
+    def mse(net):
+        Yh = net.forward(net.features(Tv, Pv, Yv))[:, :net.nout]
+        return ((Yh - net.targets(Wv)) ** 2).mean().item()
+
+    def normwise_median(net):
+        out = torch.empty_like(Wv)
+        net.predict(Tv, Pv, Yv, out)
+        return metrics.normwise_err(out, Wv).median().item()
+
+    m0, m1 = mse(untrained), mse(trained)
+    n0, n1 = normwise_median(untrained), normwise_median(trained)
+    print('arcsinh mse untrained', m0, 'trained', m1,
+          '| normwise median untrained', n0, 'trained', n1)
+    assert m1 < 0.1 * m0
+    assert n1 < n0
+    S = trained.gross_scale(Tv, Pv, Yv)
+    _r, admitted, curve = trained.calibrate_gate(Tv, Pv, Yv, Wv, S, 0.3,
+                                                 metric='normwise')
+    out = torch.empty_like(Wv)
+    m = trained.evaluate(Tv, Pv, Yv, out)
+    flags = trained._bufs['flags'][:Tv.numel()].bool()
+    assert m == int(flags.sum().item())
+    assert torch.equal(out[:, flags], Wv[:, flags])
+    ok = ~flags
+    if ok.any():
+        err = metrics.normwise_err(out[:, ok], Wv[:, ok])
+        print('admitted', ok.double().mean().item(), 'max normwise',
+              err.max().item())
+    print('calibration curve (normwise)', curve)
+
+
 def test_checkpoint_roundtrip(tmp_path):
     net = _net(seed=17)
     Td, Pd, Yd, W = _labelled('mech13', 1 << 13, 53)
