@@ -79,7 +79,8 @@ def lib_path(name):
 def build_mechanism(name, verbose=False, extra_flags=()):
     """Emit and compile synthetic mechanism ``name``; return the .so path."""
     mpath, tpath = workloads.mechanism_files(name)
-    _e, specs, reacs, fwd = load_mechanism(mpath, tpath)
+    _e, specs, reacs, fwd = load_mechanism(
+        mpath, tpath, last_spec=workloads.last_species(name))
     out_dir = os.path.join(workloads.BUILD_ROOT, name)
     src = hip_emitter.write_hip(out_dir, specs, reacs, name=name,
                                 spec_order=fwd)

@@ -137,6 +137,8 @@ class MechIR:
     rxns: List[RxnIR]
     species_names: List[str]
     nasa: List[Tuple[float, list, list]] = field(default_factory=list)
+    mw: List[float] = field(default_factory=list)        # {:.16e} literals
+    ru_mw: List[float] = field(default_factory=list)     # RU / mw, {:.16e}
 
 
 def _eff_terms(rxn):
@@ -265,7 +267,9 @@ def build_ir(specs, reacs, name='mech'):
                   rxns=rxns, species_names=[sp.name for sp in specs],
                   nasa=[(emitted(py2_str(sp.Trange[1])),
                          [float(v) for v in sp.lo], [float(v) for v in sp.hi])
-                        for sp in specs])
+                        for sp in specs],
+                  mw=[emitted(fmt_e16(sp.mw)) for sp in specs],
+                  ru_mw=[emitted(fmt_e16(chem.RU / sp.mw)) for sp in specs])
 
 
 # ---------------------------------------------------------------------------
@@ -1122,7 +1126,54 @@ def _mfma_contract_lines(c, nst):
     return out
 
 
-def _fused_fast_lines(ir, K, opts, mixed=False):
+def _nasa_masked(ir, K, k, tmids, coef_of, powers):
+    """sum_b c_b x_b over the masked basis of species k's T_mid group;
+    coef_of(a) gives the coefficients for NASA set a, powers the basis
+    indices they multiply."""
+    sp = ir.nasa[k]
+    gi = tmids.index(sp[0])
+    terms = []
+    for half, a in (('l', sp[1]), ('h', sp[2])):
+        for bi, cv in zip(powers, coef_of(a)):
+            if cv != 0.0:
+                terms.append('{} * p{}{}_{}'.format(K(cv), half, gi, bi))
+    return ' + '.join(terms) if terms else '0.0'
+
+
+# basis indices of 1, T, T^2, T^3, T^4, T^5 in the masked NASA basis
+_B1, _BT, _BT2, _BT3, _BT4, _BT5 = 0, 2, 3, 4, 5, 7
+
+
+def _dydt_cp_lines(ir, K, tmids):
+    """cp_avg = sum_k cp_k Y_k with cp_k = RU/W_k (a0 + a1 T + ... + a4 T^4)
+    (rate_subs.py:1707-1776 and the cp_avg sum of dydt, :1890-1905)."""
+    out = ['double cp_avg = 0.0;']
+    for k in range(ir.nsp):
+        poly = _nasa_masked(ir, K, k, tmids, lambda a: a[0:5],
+                            (_B1, _BT, _BT2, _BT3, _BT4))
+        yk = 'y[{}]'.format(k) if k < ir.nsp - 1 else 'y_N'
+        out.append('cp_avg = cp_avg + {} * ({}) * {};'.format(
+            K(ir.ru_mw[k]), poly, yk))
+    return out
+
+
+def _dydt_finish_lines(ir, K, tmids, k, wk):
+    """Species k's rate wk is final: its term of the energy equation
+    (w_k h_k W_k, h_k = RU/W_k (a5 + a0 T + a1 T^2/2 + ... + a4 T^5/5),
+    rate_subs.py:1495-1565) and, for k < NSP-1, dY_k/dt = w_k W_k / rho."""
+    poly = _nasa_masked(
+        ir, K, k, tmids,
+        lambda a: [a[5], a[0], a[1] / 2.0, a[2] / 3.0, a[3] / 4.0, a[4] / 5.0],
+        (_B1, _BT, _BT2, _BT3, _BT4, _BT5))
+    out = ['esum = esum + ({w}) * ({ru} * ({p})) * {mw};'.format(
+        w=wk, ru=K(ir.ru_mw[k]), p=poly, mw=K(ir.mw[k]))]
+    if k < ir.nsp - 1:
+        out.append('W[{r} * ld + i] = ({w}) * ({mw} * irho);'.format(
+            r=k + 1, w=wk, mw=K(ir.mw[k])))
+    return out
+
+
+def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
     """Per-state body of the fused FP64 kernel, arranged for the GPU.
 
     Same physics and reaction order as the reference, with the arithmetic
@@ -1140,16 +1191,24 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
     """
     nsp = ir.nsp
     ru = _lit(emitted(fmt_e8(chem.RU)))
-    lines = ['const double T = T_[i];', 'const double P = P_[i];',
-             'double y[{}];'.format(max(nsp - 1, 1))]
-    lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k) for k in range(nsp - 1)]
+    if dydt:
+        # the reference's state vector: y[0] = T, y[1 + k] = Y_k
+        lines = ['const double T = Y[i];', 'const double P = P_[i];',
+                 'double y[{}];'.format(max(nsp - 1, 1))]
+        lines += ['y[{k}] = Y[{r} * ld + i];'.format(k=k, r=k + 1)
+                  for k in range(nsp - 1)]
+    else:
+        lines = ['const double T = T_[i];', 'const double P = P_[i];',
+                 'double y[{}];'.format(max(nsp - 1, 1))]
+        lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k)
+                  for k in range(nsp - 1)]
     lines += _conc_lines(ir, K)
     lines += ['const double logT = log(T);',
               'const double iT = 1.0 / T;',
               'const double T2 = T * T;', 'const double T3 = T2 * T;',
-              'const double T4 = T3 * T;',
+              'const double T4 = T3 * T;', 'const double T5 = T4 * T;',
               'const double m = P / ({} * T);'.format(ru),
-              '(void)m; (void)T2; (void)T3; (void)T4;']
+              '(void)m; (void)T2; (void)T3; (void)T4; (void)T5;']
     if mixed:
         lines += ['const float mf = (float)m;', 'const float Tf = (float)T;',
                   '(void)mf; (void)Tf;']
@@ -1160,18 +1219,22 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
         for tm, _lo, _hi in x.kc_groups:
             if tm not in tmids:
                 tmids.append(tm)
-    if opts.get('kc') == 'species':
+    if opts.get('kc') in ('species', 'species_hg') or dydt:
         for tm, _lo, _hi in ir.nasa:
             if tm not in tmids:
                 tmids.append(tm)
     # group index of a reaction's g-th Tmid group -> global mask index
+    basis = _BASIS + (('T5',) if dydt else ())
     for gi, tm in enumerate(tmids):
         lines.append('const bool lo{g} = T <= {tm};'.format(g=gi, tm=_lit(tm)))
-        for bi, bname in enumerate(_BASIS):
+        for bi, bname in enumerate(basis):
             lines.append('const double pl{g}_{b} = lo{g} ? {v} : 0.0;'.format(
                 g=gi, b=bi, v=bname))
             lines.append('const double ph{g}_{b} = lo{g} ? 0.0 : {v};'.format(
                 g=gi, b=bi, v=bname))
+    if dydt:
+        lines += _dydt_cp_lines(ir, K, tmids)
+        lines += ['const double irho = 1.0 / rho;', 'double esum = 0.0;']
     hg = opts.get('kc') == 'species_hg'
     species_kc = opts.get('kc') in ('species', 'species_hg')
     if species_kc:
@@ -1217,7 +1280,11 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
         lines.append('double w[{}];'.format(nsp))
         for k in range(nsp):
             if k not in last:
-                lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
+                if dydt:
+                    if k < nsp - 1:
+                        lines.append('W[{r} * ld + i] = 0.0;'.format(r=k + 1))
+                else:
+                    lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
     for pos, x in enumerate(rxns):
         inner, xf = _fast_arr(x.arr, K, 'kf', expfn)
         if hg and _hg_ok(x):
@@ -1240,7 +1307,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
                 lines += _mfma_chunk_end(pos, nst)
             for k, _nu in x.spec:
                 if last.get(k) == x.index:
-                    lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
+                    lines += (_dydt_finish_lines(ir, K, tmids, k,
+                                                 'w[{}]'.format(k))
+                              if dydt else
+                              ['W[{k} * ld + i] = w[{k}];'.format(k=k)])
             continue
         inner.append('const double f = {}kf;'.format(_conc_prod(x.reac)))
         if x.rev:
@@ -1300,9 +1370,15 @@ def _fused_fast_lines(ir, K, opts, mixed=False):
             lines += _mfma_chunk_end(pos, nst)
         for k, _nu in x.spec:
             if last.get(k) == x.index:
-                lines.append('W[{k} * ld + i] = w[{k}];'.format(k=k))
+                lines += (_dydt_finish_lines(ir, K, tmids, k,
+                                             'w[{}]'.format(k))
+                          if dydt else
+                          ['W[{k} * ld + i] = w[{k}];'.format(k=k)])
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
+    if dydt:
+        # energy equation, rate_subs.py:1910-1925
+        lines.append('W[i] = (-1.0 / (rho * cp_avg)) * esum;')
     if mfma:
         # D[st][ct]: lane l holds species 16 st + 4 (l / 16) + t of the
         # state in lane 16 ct + l % 16 of this wave
@@ -1473,6 +1549,23 @@ def emit_hip_source(ir, opts=None):
         '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
         '    if (i < 0 || i >= ld) return;\n'
         '    wdot_f64_state(K, T_, P_, Y, W, ld, i);\n}\n\n')
+
+    # ---- constant-pressure dydt (FP64), the caller of the path -------
+    body.append(
+        '__device__ __forceinline__ void dydt_f64_state(\n'
+        '    const double *__restrict__ K, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Y, double *__restrict__ W,\n'
+        '    int64_t ld, int64_t i)\n{\n' +
+        _indent(_fused_fast_lines(ir, K, dict(opts, contraction='sparse'),
+                                  dydt=True), 4) + '}\n\n')
+    body.append(
+        '__global__ __launch_bounds__(crs::kBlock) void k_dydt_f64(\n'
+        '    const double *__restrict__ K, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
+        '    int64_t ld)\n{\n'
+        '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
+        '    if (i >= n) return;\n'
+        '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
 
     # ---- fused mixed-precision per-state body -------------------------
     if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
@@ -1704,6 +1797,26 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
     CRS_TRY(table(fn, &K));
 {launch_full}    k_wdot_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, P, Y, wdot,
                                                         nullptr, n, ld);
+    return crs::launched(fn);
+}}
+
+int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
+             int64_t ld, int mode, void *stream)
+{{
+    const char *fn = "crs_dydt";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (mode != CRS_MODE_FP64)
+        return crs::fail(mode == CRS_MODE_MIXED ? CRS_ERR_UNSUPPORTED
+                                                : CRS_ERR_ARG,
+                         "%s: mode %d not available (FP64 only)", fn, mode);
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, y, "y", n));
+    CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    k_dydt_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, P, y, dy, n, ld);
     return crs::launched(fn);
 }}
 

@@ -23,7 +23,7 @@ MODES = {'fp64': 0, 'mixed': 1}
 EXPORTED_SYMBOLS = ('crs_get_mech_info', 'crs_last_error', 'crs_eval_conc',
                     'crs_eval_rxn_rates', 'crs_get_rxn_pres_mod',
                     'crs_eval_spec_rates', 'crs_eval_wdot',
-                    'crs_eval_wdot_subset')
+                    'crs_eval_wdot_subset', 'crs_dydt')
 
 
 class CrsError(RuntimeError):
@@ -48,6 +48,7 @@ _SIGS = {
     'crs_eval_spec_rates': [_P] * 5 + [_I64, _I64, _P],
     'crs_eval_wdot': [_P] * 4 + [_I64, _I64, ctypes.c_int, _P],
     'crs_eval_wdot_subset': [_P] * 5 + [_I64, _I64, ctypes.c_int, _P],
+    'crs_dydt': [_P] * 3 + [_I64, _I64, ctypes.c_int, _P],
 }
 
 
@@ -173,6 +174,25 @@ class MechLib(object):
         self._check(self.lib.crs_eval_wdot(_ptr(T), _ptr(P), _ptr(Y), _ptr(W),
                                            n, ld, MODES[mode], _stream()))
         return W
+
+    def dydt(self, P, y, out=None, mode='fp64'):
+        """Constant-pressure RHS (the reference's dydt, CONP): y [>= NSP][ld]
+        with row 0 = T and rows 1..NSP-1 the mass fractions; returns dy of
+        the same shape (rows beyond NSP-1 untouched)."""
+        n = P.numel()
+        self._vec(P, n, 'P')
+        rows = y.shape[0] if y.dim() == 2 else -1
+        if rows not in (self.nsp, self.nsp + 1):
+            raise ValueError('y must have NSP or NSP+1 rows')
+        ld = self._rows(y, rows, n, 'y')
+        dy = out if out is not None else self._new(rows, n, ld)
+        if self._rows(dy, dy.shape[0], n, 'dy') != ld or \
+                dy.shape[0] not in (self.nsp, self.nsp + 1):
+            raise ValueError('dy must have NSP or NSP+1 rows and the row '
+                             'stride of y')
+        self._check(self.lib.crs_dydt(_ptr(P), _ptr(y), _ptr(dy), n, ld,
+                                      MODES[mode], _stream()))
+        return dy
 
     def eval_wdot_subset(self, T, P, Y, W, idx, mode='fp64'):
         n = T.numel()

@@ -20,13 +20,26 @@ from dataclasses import dataclass
 
 from . import synth
 
-__all__ = ['MECHANISMS', 'WORKLOADS', 'Workload', 'mechanism_files',
+__all__ = ['MECHANISMS', 'REORDERED', 'LIBRARIES', 'last_species',
+           'WORKLOADS', 'Workload', 'mechanism_files',
            'BUILD_ROOT']
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD_ROOT = os.path.join(HERE, '_build')
 
 MECHANISMS = synth.MECHS
+
+# Libraries built from a synthetic mechanism with another species moved to
+# the end (the generator's -ls option): name -> (mechanism, last species).
+# The synthetic bath gas is inert; a reactive last species exercises the
+# implied-species terms (dy_N) of the path and of dydt.
+REORDERED = {'mech13co': ('mech13', 'CO')}
+LIBRARIES = list(MECHANISMS) + list(REORDERED)
+
+
+def last_species(name):
+    """The species a library's mechanism is reordered to end with, or None."""
+    return REORDERED[name][1] if name in REORDERED else None
 
 
 @dataclass(frozen=True)
@@ -49,7 +62,7 @@ WORKLOADS = {
 
 def mechanism_files(name, directory=None):
     """Write (if needed) and return the Chemkin/thermo paths of a mechanism."""
-    cfg = MECHANISMS[name]
+    cfg = MECHANISMS[REORDERED[name][0] if name in REORDERED else name]
     directory = directory or os.path.join(BUILD_ROOT, 'mechs')
     mpath = os.path.join(directory, cfg.name + '.dat')
     tpath = os.path.join(directory, cfg.name + '_therm.dat')

@@ -122,6 +122,21 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
                          double *wdot, const int32_t *idx, int64_t m,
                          int64_t ld, int mode, void *stream);
 
+/*
+ * Batched constant-pressure right-hand side.  Replaces `void dydt(const
+ * double t, const double pres, const double *y, double *dy)` emitted by
+ * rate_subs.py:1829-1946 (CONP branch; prototype rate_subs.py:1808-1816):
+ * y and dy are [rows][ld] with row 0 = T [K] and rows 1..NSP-1 the mass
+ * fractions of species 0..NSP-2 (the last species is implied); P[n] [Pa].
+ * dy row 0 = dT/dt = -sum_k wdot_k h_k W_k / (rho cp), rows 1..NSP-1 =
+ * dY_k/dt = wdot_k W_k / rho.  Any row beyond NSP-1 (an integrator's NN-th
+ * slot) is neither read nor written, as in the reference.  The time
+ * argument of the reference is unused there and omitted here.  FP64 only
+ * (CRS_MODE_MIXED returns CRS_ERR_UNSUPPORTED).
+ */
+int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
+             int64_t ld, int mode, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

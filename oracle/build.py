@@ -47,7 +47,8 @@ def build(name, mech_path=None, therm_path=None, force=True):
     so = os.path.join(out, 'liboracle.so')
     if not force and os.path.exists(so):
         return so
-    _e, specs, reacs, fwd = load_mechanism(mech_path, therm_path)
+    _e, specs, reacs, fwd = load_mechanism(
+        mech_path, therm_path, last_spec=workloads.last_species(name))
     srcs = emit_c.emit_c(specs, reacs, os.path.join(out, 'src'), fwd)
     cmd = (['gcc'] + CFLAGS + ['-I', os.path.join(out, 'src'), '-o',
                                 so + '.tmp'] +
@@ -85,6 +86,8 @@ class Oracle(object):
                                                 [i64, i64, ctypes.c_int])
         self.lib.oracle_spec_rates.argtypes = [P, P, P, P, i64, i64,
                                                ctypes.c_int]
+        self.lib.oracle_dydt.argtypes = [P, P, P, i64, i64, ctypes.c_int]
+        self.lib.oracle_thermo.argtypes = [P, P, P, i64, i64, ctypes.c_int]
 
     @staticmethod
     def default_threads():
@@ -130,11 +133,33 @@ class Oracle(object):
         return W
 
 
+    def dydt(self, P, y, nthreads=None):
+        """Constant-pressure RHS: y [>= NSP][ld] (row 0 T, rows 1..NSP-1
+        mass fractions) -> dy [NSP][ld]."""
+        P, y = _arr(P), _arr(y)
+        n = P.size
+        ld = y.shape[1]
+        dy = np.empty((self.nsp, ld))
+        self.lib.oracle_dydt(_dptr(P), _dptr(y), _dptr(dy), n, ld,
+                             nthreads or self.default_threads())
+        return dy
+
+    def thermo(self, T, nthreads=None):
+        """(cp [NSP][n] J/kg/K, h [NSP][n] J/kg) at temperatures T."""
+        T = _arr(T)
+        n = T.size
+        cp = np.empty((self.nsp, n))
+        h = np.empty((self.nsp, n))
+        self.lib.oracle_thermo(_dptr(T), _dptr(cp), _dptr(h), n, n,
+                               nthreads or self.default_threads())
+        return cp, h
+
+
 def build_all(force=True):
-    return [build(name, force=force) for name in workloads.MECHANISMS]
+    return [build(name, force=force) for name in workloads.LIBRARIES]
 
 
 if __name__ == '__main__':
-    names = sys.argv[1:] or list(workloads.MECHANISMS)
+    names = sys.argv[1:] or list(workloads.LIBRARIES)
     for nm in names:
         print(build(nm))

@@ -110,3 +110,39 @@ void oracle_spec_rates(const double *fwd_in, const double *rev_in,
         for (int k = 0; k < NSP; ++k) W[(int64_t)k * ld + i] = sp[k];
     }
 }
+
+/*
+ * Constant-pressure right-hand side for n states: y and dy are [rows][ld]
+ * with row 0 = T, rows 1..NSP-1 = mass fractions of species 0..NSP-2 (the
+ * reference's y / dy, rate_subs.py:1829-1946); P[n] in Pa.  Rows beyond
+ * NSP-1 are neither read nor written.
+ */
+void oracle_dydt(const double *P, const double *y, double *dy, int64_t n,
+                 int64_t ld, int nthreads)
+{
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i) {
+        double ys[NSP], dys[NSP];
+        for (int k = 0; k < NSP; ++k)
+            ys[k] = y[(int64_t)k * ld + i];
+        dydt(0.0, P[i], ys, dys);
+        for (int k = 0; k < NSP; ++k)
+            dy[(int64_t)k * ld + i] = dys[k];
+    }
+}
+
+/* Species cp [J/kg/K] and h [J/kg], [NSP][ld], at n temperatures. */
+void oracle_thermo(const double *T, double *cp, double *h, int64_t n,
+                   int64_t ld, int nthreads)
+{
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i) {
+        double c[NSP], e[NSP];
+        eval_cp(T[i], c);
+        eval_h(T[i], e);
+        for (int k = 0; k < NSP; ++k) {
+            cp[(int64_t)k * ld + i] = c[k];
+            h[(int64_t)k * ld + i] = e[k];
+        }
+    }
+}

@@ -11,6 +11,10 @@ would have generated for the same mechanism:
 * ``eval_spec_rates``    rate_subs.py:1076-1235
 * ``header.h``           create_rate_subs.py:332-427 (the parts the four
   functions need)
+* ``eval_h``, ``eval_cp`` rate_subs.py:1495-1565, 1707-1776 (chem_utils.c)
+* ``dydt`` (constant pressure) rate_subs.py:1829-1946, the caller of the
+  four functions: energy equation and mass-fraction rates (emitted without
+  the reference's ``#if defined(CONP)`` guard)
 
 The reference is Python 2; every float it prints with ``str()`` or an empty
 format spec is spelled here with ``utils.py2_str`` (12 significant digits),
@@ -375,6 +379,101 @@ def _write_eval_conc(specs):
     return ''.join(out)
 
 
+def _nasa_if(sp):
+    return '  if (T <= {})'.format(py2_str(sp.Trange[1]))
+
+
+def _write_thermo(specs):
+    """eval_h and eval_cp, rate_subs.py:1495-1565 and :1707-1776."""
+    out = ['#include "header.h"\n\n',
+           'void eval_h (const double T, double * h) {\n\n']
+    for k, sp in enumerate(specs):
+        out.append(_nasa_if(sp) + ' {\n')
+        for half, a in (('lo', sp.lo), ('hi', sp.hi)):
+            out.append('    h[{}] = {} * ({} + T * ({} + T * ({} + T * ({} + '
+                       'T * ({} + {} * T)))));\n'.format(
+                           k, fmt_e16(chem.RU / sp.mw), fmt_e16(a[5]),
+                           fmt_e16(a[0]), fmt_e16(a[1] / 2.0),
+                           fmt_e16(a[2] / 3.0), fmt_e16(a[3] / 4.0),
+                           fmt_e16(a[4] / 5.0)))
+            if half == 'lo':
+                out.append('  } else {\n')
+        out.append('  }\n\n')
+    out.append('} // end eval_h\n\n')
+    out.append('void eval_cp (const double T, double * cp) {\n\n')
+    for k, sp in enumerate(specs):
+        out.append(_nasa_if(sp) + ' {\n')
+        for half, a in (('lo', sp.lo), ('hi', sp.hi)):
+            out.append('    cp[{}] = {} * ({} + T * ({} + T * ({} + T * ({} + '
+                       '{} * T))));\n'.format(
+                           k, fmt_e16(chem.RU / sp.mw), fmt_e16(a[0]),
+                           fmt_e16(a[1]), fmt_e16(a[2]), fmt_e16(a[3]),
+                           fmt_e16(a[4])))
+            if half == 'lo':
+                out.append('  } else {\n')
+        out.append('  }\n\n')
+    out.append('} // end eval_cp\n\n')
+    return ''.join(out)
+
+
+def _write_dydt(specs, reacs):
+    """Constant-pressure dydt, rate_subs.py:1829-1946 (C branch).
+
+    y[0] = T, y[1..NSP-1] = mass fractions of species 0..NSP-2; dy alike;
+    the last species is implied and its slot is not written.
+    """
+    nsp = len(specs)
+    nr = len(reacs)
+    nrev = sum(1 for r in reacs if r.rev)
+    npd = sum(1 for r in reacs if r.thd_body or r.pdep)
+    out = ['#include "header.h"\n#include "chem_utils.h"\n#include "rates.h"\n\n',
+           'void dydt (const double t, const double pres, '
+           'const double * y, double * dy) {\n\n',
+           '  // species molar concentrations\n',
+           '  double conc[{}];\n'.format(nsp),
+           '  double y_N;\n  double mw_avg;\n  double rho;\n',
+           '  eval_conc (y[0], pres, &y[1], &y_N, &mw_avg, &rho, conc);\n\n',
+           '  // local arrays holding reaction rates\n',
+           '  double fwd_rates[{}];\n'.format(nr)]
+    if nrev:
+        out.append('  double rev_rates[{}];\n'.format(nrev))
+    else:
+        out.append('  double* rev_rates = 0;\n')
+    out.append('  eval_rxn_rates (y[0], pres, conc, fwd_rates, rev_rates);\n\n')
+    if npd:
+        out.append('  // get pressure modifications to reaction rates\n'
+                   '  double pres_mod[{}];\n'.format(npd) +
+                   '  get_rxn_pres_mod (y[0], pres, conc, pres_mod);\n')
+    else:
+        out.append('  double* pres_mod = 0;\n')
+    out.append('\n')
+    out.append('  // evaluate species molar net production rates\n'
+               '  double dy_N;\n'
+               '  eval_spec_rates (fwd_rates, rev_rates, pres_mod, &dy[1], '
+               '&dy_N);\n\n')
+    out.append('  // local array holding constant pressure specific heat\n'
+               '  double cp[{}];\n'.format(nsp) +
+               '  eval_cp (y[0], cp);\n\n')
+    out.append('  // constant pressure mass-average specific heat\n')
+    terms = ['(cp[{}] * y[{}])'.format(k, k + 1) for k in range(nsp - 1)]
+    terms.append('(cp[{}] * y_N)'.format(nsp - 1))
+    out.append('  double cp_avg = ' + ' + '.join(terms) + ';\n\n')
+    out.append('  // local array for species enthalpies\n'
+               '  double h[{}];\n'.format(nsp) + '  eval_h(y[0], h);\n')
+    out.append('  // rate of change of temperature\n')
+    terms = ['(dy[{}] * h[{}] * {})'.format(k + 1, k, fmt_e16(specs[k].mw))
+             for k in range(nsp - 1)]
+    terms.append('(dy_N * h[{}] * {})'.format(nsp - 1, fmt_e16(specs[-1].mw)))
+    out.append('  dy[0] = (-1.0 / (rho * cp_avg)) * (' + ' + '.join(terms) +
+               ');\n\n')
+    out.append('  // calculate rate of change of species mass fractions\n')
+    for k in range(nsp - 1):
+        out.append('  dy[{}] *= ({} / rho);\n'.format(k + 1,
+                                                      fmt_e16(specs[k].mw)))
+    out.append('\n} // end dydt\n\n')
+    return ''.join(out)
+
+
 def _write_headers(specs, reacs):
     nsp = len(specs)
     pdep = any(r.thd_body or r.pdep for r in reacs)
@@ -397,7 +496,11 @@ def _write_headers(specs, reacs):
     chem_h = ('#ifndef CHEM_UTILS_HEAD\n#define CHEM_UTILS_HEAD\n\n'
               '#include "header.h"\n\n'
               'void eval_conc (const double, const double, const double*, '
-              'double*, double*, double*, double*);\n\n#endif\n')
+              'double*, double*, double*, double*);\n'
+              'void eval_h (const double, double*);\n'
+              'void eval_cp (const double, double*);\n'
+              'void dydt (const double, const double, const double*, '
+              'double*);\n\n#endif\n')
     return header, rates, chem_h
 
 
@@ -428,6 +531,8 @@ def emit_c(specs, reacs, outdir, spec_order=None):
         'chem_utils.c': _write_eval_conc(specs),
         'rxn_rates.c': _write_rxn_rates(specs, reacs),
         'spec_rates.c': _write_spec_rates(specs, reacs, spec_order),
+        'thermo.c': _write_thermo(specs),
+        'dydt.c': _write_dydt(specs, reacs),
     }
     if npd:
         files['rxn_rates_pres_mod.c'] = _write_pres_mod(specs, reacs)

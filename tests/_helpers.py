@@ -15,7 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def mechanism(name):
     """(specs, reacs, fwd_spec_mapping) of a synthetic mechanism."""
     mpath, tpath = workloads.mechanism_files(name)
-    _e, specs, reacs, fwd = load_mechanism(mpath, tpath)
+    _e, specs, reacs, fwd = load_mechanism(
+        mpath, tpath, last_spec=workloads.last_species(name))
     return specs, reacs, fwd
 
 

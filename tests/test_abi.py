@@ -23,7 +23,7 @@ def test_header_declares_exactly_the_runtime_symbols():
 
 def _libs():
     out = []
-    for name in workloads.MECHANISMS:
+    for name in workloads.LIBRARIES:
         p = build.lib_path(name)
         if os.path.exists(p):
             out.append((name, p))
