@@ -181,6 +181,24 @@ def large_section(args, torch):
     return out
 
 
+def dydt_section(args, torch, lib, T, P, Y):
+    """Constant-pressure dydt (the caller of the path) on the headline
+    states: the same fused evaluation plus cp, h, the energy equation and
+    the mass-fraction rates."""
+    n = T.numel()
+    nsp = lib.nsp
+    y = torch.empty((nsp, n), dtype=torch.float64, device='cuda')
+    y[0] = T
+    y[1:] = Y[:nsp - 1]
+    dy = torch.empty_like(y)
+    steps = max(3, args.steps // 2)
+    ms, kern = timed(torch, lambda: lib.dydt(P, y, out=dy), steps, 1)
+    return dict(workload='w1_53 states in the reference dydt layout '
+                         '(row 0 = T, rows 1..NSP-1 = Y)',
+                states_per_s=n / (ms * 1e-3), kernel_ms=kern,
+                kernel='k_dydt_f64')
+
+
 def _quantiles(torch, x, qs=(0.01, 0.1, 0.5, 0.9, 0.99)):
     """Quantiles of a per-state error vector (strided sample if large)."""
     x = x[::max(1, x.numel() >> 20)].double()
@@ -407,6 +425,8 @@ def main():
         line['mixed'] = mixed
         log('mixed: {:.3e} states/s, speed-up {:.2f}'.format(
             mixed['states_per_s'], mixed['speedup_vs_fp64']))
+    line['dydt'] = dydt_section(args, torch, lib, Td, Pd, Yd)
+    log('dydt: {:.3e} states/s'.format(line['dydt']['states_per_s']))
     if not args.no_large:
         line['large'] = large_section(args, torch)
         log('large: {:.3e} states/s fp64'.format(

@@ -123,6 +123,8 @@ class RxnIR:
     falloff_high: bool = False               # HIGH given (chem. activated)
     troe: Optional[dict] = None
     sri: Optional[dict] = None
+    plog: Optional[dict] = None              # rate_subs.py:446-485
+    cheb: Optional[dict] = None              # rate_subs.py:138-223
     spec: List[Tuple[int, float]] = field(default_factory=list)  # (k, nu)
 
 
@@ -206,12 +208,29 @@ def build_ir(specs, reacs, name='mech'):
     pd_reacs = [j for j, r in enumerate(reacs) if r.thd_body or r.pdep]
     rxns = []
     for j, r in enumerate(reacs):
-        if r.cheb or r.plog:
-            raise NotImplementedError('PLOG/Chebyshev reactions are not '
-                                      'supported by the HIP emitter')
         x = RxnIR(index=j, reac=list(zip(r.reac, r.reac_nu)),
                   prod=list(zip(r.prod, r.prod_nu)),
                   arr=arr_ir(r.A, r.b, r.E))
+        if r.plog:
+            rows = r.plog_par
+            x.plog = dict(
+                p=[emitted('{:.4e}'.format(v[0])) for v in rows],
+                arr=[arr_ir(v[1], v[2], v[3]) for v in rows],
+                lnp=[emitted(fmt_e16(math.log(v[0]))) for v in rows],
+                dlnp=[emitted(fmt_e16(math.log(rows[i + 1][0]) -
+                                      math.log(rows[i][0])))
+                      for i in range(len(rows) - 1)])
+        if r.cheb:
+            t0, t1 = r.cheb_tlim
+            p0, p1 = r.cheb_plim
+            x.cheb = dict(
+                nt=r.cheb_n_temp, np=r.cheb_n_pres,
+                par=[[emitted(fmt_e8(float(v))) for v in row]
+                     for row in r.cheb_par],
+                tsum=emitted(fmt_e8(1.0 / t0 + 1.0 / t1)),
+                tsub=emitted(fmt_e8(1.0 / t1 - 1.0 / t0)),
+                psum=emitted(fmt_e8(math.log10(p0) + math.log10(p1))),
+                psub=emitted(fmt_e8(math.log10(p1) - math.log10(p0))))
         if r.rev:
             x.rev = True
             x.rev_index = rev_reacs.index(j)
@@ -434,9 +453,133 @@ def _pres_mod_lines(x, K, kf_main='kf', C='C'):
     return out
 
 
+def _plog_lines(x, K, P='P'):
+    """PLOG kf as the reference writes it (rate_subs.py:446-485)."""
+    pl = x.plog
+    n = len(pl['p'])
+    out = ['double kf;', 'double kf2;',
+           'if ({} <= {}) {{'.format(P, _lit(pl['p'][0])),
+           '    kf = {};'.format(_arr_expr(pl['arr'][0], K))]
+    for i in range(n - 1):
+        out.append('}} else if (({P} > {a}) && ({P} <= {b})) {{'.format(
+            P=P, a=_lit(pl['p'][i]), b=_lit(pl['p'][i + 1])))
+        out.append('    kf = log({});'.format(_arr_expr(pl['arr'][i], K)))
+        out.append('    kf2 = log({});'.format(_arr_expr(pl['arr'][i + 1], K)))
+        out.append('    kf = exp(kf + (kf2 - kf) * (log({}) - {}) / {});'.format(
+            P, K(pl['lnp'][i]), K(pl['dlnp'][i])))
+    out.append('}} else if ({} > {}) {{'.format(P, _lit(pl['p'][-1])))
+    out.append('    kf = {};'.format(_arr_expr(pl['arr'][-1], K)))
+    out.append('}')
+    out.append('(void)kf2;')
+    return out
+
+
+def _cheb_lines(x, K, P='P'):
+    """Chebyshev kf as the reference writes it (rate_subs.py:138-223),
+    with the CUDA branch's exp10 in place of C's pow(10.0, .)."""
+    c = x.cheb
+    out = ['double kf;',
+           'const double Tred = ((2.0 / T) - {}) / {};'.format(K(c['tsum']),
+                                                               K(c['tsub'])),
+           'const double Pred = (2.0 * log10({}) - {}) / {};'.format(
+               P, K(c['psum']), K(c['psub'])),
+           'double cheb_temp_0 = 1;', 'double cheb_temp_1 = Pred;',
+           'double dot_prod[{}];'.format(c['nt'])]
+    for i in range(c['nt']):
+        out.append('dot_prod[{}] = {} + Pred * {};'.format(
+            i, K(c['par'][i][0]), K(c['par'][i][1])))
+    one = True
+    for j in range(2, c['np']):
+        new, old = (1, 0) if one else (0, 1)
+        out.append('cheb_temp_{o} = 2 * Pred * cheb_temp_{n} - cheb_temp_{o};'
+                   .format(o=old, n=new))
+        for i in range(c['nt']):
+            out.append('dot_prod[{}] += {} * cheb_temp_{};'.format(
+                i, K(c['par'][i][j]), old))
+        one = not one
+    out += ['cheb_temp_0 = 1;', 'cheb_temp_1 = Tred;',
+            'kf = dot_prod[0] + Tred * dot_prod[1];']
+    one = True
+    for i in range(2, c['nt']):
+        new, old = (1, 0) if one else (0, 1)
+        out.append('cheb_temp_{o} = 2 * Tred * cheb_temp_{n} - cheb_temp_{o};'
+                   .format(o=old, n=new))
+        out.append('kf += dot_prod[{}] * cheb_temp_{};'.format(i, old))
+        one = not one
+    out.append('kf = exp10(kf);')
+    return out
+
+
+def _fast_exponent(a, K):
+    """ln of an Arrhenius expression (lnA + b lnT - E/T with 1/T
+    precomputed; the log of the literal forms without an exponential)."""
+    if not a.has_exp or a.pre is not None:
+        return 'log({})'.format(_arr_expr(a, K))
+    lines, xname = _fast_arr(a, K, 'tmp')
+    return lines[0][len('const double x_tmp = '):-1]
+
+
+def _plog_fast_lines(x, K, expfn):
+    """PLOG kf for the fused kernels: the bracketing rows' ln k are
+    interpolated in ln P directly (no exp/log round trip) and one
+    exponential is taken, same thresholds as the reference."""
+    pl = x.plog
+    n = len(pl['p'])
+    xs = [_fast_exponent(a, K) for a in pl['arr']]
+    out = ['double xk;',
+           'if (P <= {}) {{'.format(_lit(pl['p'][0])),
+           '    xk = {};'.format(xs[0])]
+    for i in range(n - 1):
+        out.append('}} else if (P <= {}) {{'.format(_lit(pl['p'][i + 1])))
+        out.append('    const double x0 = {};'.format(xs[i]))
+        out.append('    const double x1 = {};'.format(xs[i + 1]))
+        out.append('    xk = x0 + (x1 - x0) * ((logP - {}) * {});'.format(
+            K(pl['lnp'][i]), K(1.0 / pl['dlnp'][i])))
+    out += ['} else {', '    xk = {};'.format(xs[-1]), '}',
+            'const double kf = {}(xk);'.format(expfn)]
+    return out
+
+
+def _cheb_fast_lines(x, K, expfn):
+    """Chebyshev kf for the fused kernels: reduced coordinates from 1/T and
+    log10 P computed once per state, 10^s as exp(s ln 10)."""
+    c = x.cheb
+    out = ['const double Tred = (2.0 * iT - {}) * {};'.format(
+               K(c['tsum']), K(1.0 / c['tsub'])),
+           'const double Pred = (2.0 * log10P - {}) * {};'.format(
+               K(c['psum']), K(1.0 / c['psub']))]
+    # sum_i T_i(Tred) sum_j c_ij T_j(Pred), recurrences as the reference
+    out.append('double tp0 = 1.0, tp1 = Pred;')
+    dots = ['{} + Pred * {}'.format(K(c['par'][i][0]), K(c['par'][i][1]))
+            for i in range(c['nt'])]
+    out += ['double d{} = {};'.format(i, d) for i, d in enumerate(dots)]
+    one = True
+    for j in range(2, c['np']):
+        new, old = (1, 0) if one else (0, 1)
+        out.append('tp{o} = 2.0 * Pred * tp{n} - tp{o};'.format(o=old, n=new))
+        for i in range(c['nt']):
+            out.append('d{i} += {c} * tp{o};'.format(i=i, c=K(c['par'][i][j]),
+                                                     o=old))
+        one = not one
+    out += ['double tt0 = 1.0, tt1 = Tred;', 'double lk = d0 + Tred * d1;']
+    one = True
+    for i in range(2, c['nt']):
+        new, old = (1, 0) if one else (0, 1)
+        out.append('tt{o} = 2.0 * Tred * tt{n} - tt{o};'.format(o=old, n=new))
+        out.append('lk += d{} * tt{};'.format(i, old))
+        one = not one
+    out.append('const double kf = {}(lk * 2.302585092994046);'.format(expfn))
+    return out
+
+
 def _rate_lines(x, K, C='C'):
     """Lines for kf, f and (if reversible) Kc and r of reaction x."""
-    out = ['const double kf = {};'.format(_arr_expr(x.arr, K))]
+    if x.plog:
+        out = _plog_lines(x, K)
+    elif x.cheb:
+        out = _cheb_lines(x, K)
+    else:
+        out = ['const double kf = {};'.format(_arr_expr(x.arr, K))]
     out.append('const double f = {}kf;'.format(_conc_prod(x.reac, C)))
     if x.rev:
         if x.rev_arr is not None:
@@ -1209,6 +1352,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
               'const double T4 = T3 * T;', 'const double T5 = T4 * T;',
               'const double m = P / ({} * T);'.format(ru),
               '(void)m; (void)T2; (void)T3; (void)T4; (void)T5;']
+    if any(x.plog or x.cheb for x in ir.rxns):
+        lines += ['const double logP = log(P);',
+                  'const double log10P = logP * 0.43429448190325182;',
+                  '(void)log10P;']
     if mixed:
         lines += ['const float mf = (float)m;', 'const float Tf = (float)T;',
                   '(void)mf; (void)Tf;']
@@ -1286,7 +1433,12 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
                 else:
                     lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
     for pos, x in enumerate(rxns):
-        inner, xf = _fast_arr(x.arr, K, 'kf', expfn)
+        if x.plog:
+            inner, xf = _plog_fast_lines(x, K, expfn), None
+        elif x.cheb:
+            inner, xf = _cheb_fast_lines(x, K, expfn), None
+        else:
+            inner, xf = _fast_arr(x.arr, K, 'kf', expfn)
         if hg and _hg_ok(x):
             pm = ''
             if x.pd_index is not None:
@@ -1452,6 +1604,8 @@ def emit_hip_source(ir, opts=None):
     lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
              'if (i >= n) return;',
              'const double T = T_[i];',
+             'const double P = P_[i];',
+             '(void)P;',
              'const double logT = log(T);',
              'double C[{}];'.format(nsp)]
     lines += ['C[{k}] = Cin[{k} * ld + i];'.format(k=k) for k in range(nsp)]
@@ -1466,7 +1620,8 @@ def emit_hip_source(ir, opts=None):
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_eval_rxn_rates(\n'
         '    const double *__restrict__ K,\n'
-        '    const double *__restrict__ T_, const double *__restrict__ Cin,\n'
+        '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+        '    const double *__restrict__ Cin,\n'
         '    double *__restrict__ fwd, double *__restrict__ rev, int64_t n,\n'
         '    int64_t ld)\n{\n' + _indent(lines, 4) + '}\n\n')
 
@@ -1719,9 +1874,9 @@ int crs_eval_rxn_rates(const double *T, const double *P, const double *C,
                        void *stream)
 {{
     const char *fn = "crs_eval_rxn_rates";
-    (void)P;
     CRS_TRY(crs::check_sizes(fn, n, ld));
     CRS_TRY(crs::check_ptr(fn, T, "T", n));
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
     CRS_TRY(crs::check_ptr(fn, C, "C", n));
     CRS_TRY(crs::check_ptr(fn, fwd, "fwd", n));
     CRS_TRY(crs::check_ptr(fn, rev, "rev", NREV > 0 ? n : 0));
@@ -1729,8 +1884,8 @@ int crs_eval_rxn_rates(const double *T, const double *P, const double *C,
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
-    k_eval_rxn_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, C, fwd, rev,
-                                                              n, ld);
+    k_eval_rxn_rates<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, P, C, fwd,
+                                                              rev, n, ld);
     return crs::launched(fn);
 }}
 

@@ -47,6 +47,8 @@ class MechConfig:
     frac_troe: float = 0.0
     frac_lind: float = 0.0
     frac_sri: float = 0.0
+    frac_plog: float = 0.0
+    frac_cheb: float = 0.0
     seed: int = 0
 
 
@@ -57,7 +59,15 @@ MECHS = {
                          frac_lind=0.02, seed=1001),
     'mech111': MechConfig('mech111', 111, 784, 0.80, 0.12, 5, frac_troe=0.10,
                           frac_sri=0.03, seed=1003),
+    # pressure-dependent rate forms (PLOG, Chebyshev) beside the others
+    'mech21pc': MechConfig('mech21pc', 21, 60, 0.80, 0.10, 3, frac_troe=0.05,
+                           frac_plog=0.15, frac_cheb=0.15, seed=1005),
 }
+
+# PLOG table pressures [atm] and Chebyshev validity box of the generator
+PLOG_ATM = (0.1, 1.0, 10.0, 100.0)
+CHEB_T = (300.0, 3000.0)
+CHEB_P_ATM = (0.01, 200.0)
 
 
 def _species_name(comp):
@@ -141,12 +151,14 @@ class _Rxn:
     reac: list
     prod: list
     rev: bool
-    kind: str = 'elem'          # elem | thd | troe | lind | sri
+    kind: str = 'elem'          # elem | thd | troe | lind | sri | plog | cheb
     arr: tuple = ()
     low: tuple = ()
     troe: tuple = ()
     sri: tuple = ()
     eff: list = field(default_factory=list)
+    plog: list = field(default_factory=list)   # (P atm, A, b, Ea) rows
+    cheb: list = field(default_factory=list)   # [n_temp][n_pres] coeffs
 
 
 def _side_text(terms, names):
@@ -216,8 +228,10 @@ def generate_mechanism(cfg):
     n_troe = round(cfg.frac_troe * nr)
     n_lind = round(cfg.frac_lind * nr)
     n_sri = round(cfg.frac_sri * nr)
+    n_plog = round(cfg.frac_plog * nr)
+    n_cheb = round(cfg.frac_cheb * nr)
     kinds = (['thd'] * n_thd + ['troe'] * n_troe + ['lind'] * n_lind +
-             ['sri'] * n_sri)
+             ['sri'] * n_sri + ['plog'] * n_plog + ['cheb'] * n_cheb)
     kinds += ['elem'] * (nr - len(kinds))
     kinds = [kinds[i] for i in rng.permutation(nr)]
     rev_flags = np.zeros(nr, dtype=bool)
@@ -227,7 +241,26 @@ def generate_mechanism(cfg):
     for j, (reac, prod) in enumerate(balanced):
         r = _Rxn(reac=reac, prod=prod, rev=bool(rev_flags[j]), kind=kinds[j])
         r.arr = _arrhenius(rng)
-        if r.kind != 'elem':
+        if r.kind == 'plog':
+            # ln k varies smoothly with ln P: a shifted A per pressure
+            A, b, Ea = r.arr
+            slope = rng.uniform(-0.5, 1.0)
+            for p in PLOG_ATM:
+                la = math.log10(A) + slope * math.log10(p) + rng.uniform(-0.2, 0.2)
+                r.plog.append((p, float('%.6E' % 10.0 ** la),
+                               round(b + rng.uniform(-0.2, 0.2), 3),
+                               round(Ea + rng.uniform(-500.0, 500.0), 3)))
+        if r.kind == 'cheb':
+            nt, npr = (6, 4) if rng.random() < 0.5 else (5, 3)
+            c = np.zeros((nt, npr))
+            c[0, 0] = round(rng.uniform(9.0, 13.0), 4)
+            c[1, 0] = round(-rng.uniform(0.2, 3.0), 4)
+            for i in range(nt):
+                for jp in range(npr):
+                    if (i, jp) not in ((0, 0), (1, 0)):
+                        c[i, jp] = round(rng.normal(0.0, 0.3 / (1 + i + jp)), 4)
+            r.cheb = c.tolist()
+        if r.kind not in ('elem', 'plog', 'cheb'):
             picks = rng.choice(nsp, size=cfg.n_enhanced, replace=False)
             r.eff = [(int(k), round(rng.uniform(0.5, 12.0), 2))
                      for k in sorted(picks)]
@@ -290,6 +323,18 @@ def generate_mechanism(cfg):
         if r.eff:
             mech.append('    ' + ' '.join('{}/{:.2f}/'.format(names[k], e)
                                           for k, e in r.eff) + '\n')
+        for p, pa, pb, pe in r.plog:
+            mech.append('    PLOG / {:.6E} {:.6E} {:.3f} {:.3f} /\n'.format(
+                p, pa, pb, pe))
+        if r.cheb:
+            nt, npr = len(r.cheb), len(r.cheb[0])
+            mech.append('    TCHEB / {:.1f} {:.1f} /\n'.format(*CHEB_T))
+            mech.append('    PCHEB / {:.4f} {:.1f} /\n'.format(*CHEB_P_ATM))
+            flat = ['{:.4f}'.format(v) for row in r.cheb for v in row]
+            mech.append('    CHEB / {} {} {} /\n'.format(nt, npr,
+                                                          ' '.join(flat[:4])))
+            for k in range(4, len(flat), 6):
+                mech.append('    CHEB / {} /\n'.format(' '.join(flat[k:k + 6])))
     mech.append('END\n')
     return ''.join(mech), ''.join(thermo)
 

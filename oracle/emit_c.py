@@ -23,8 +23,7 @@ format the reference uses.  It works directly on the reference's data model
 (``SpecInfo``/``ReacInfo`` attributes), not on the HIP emitter's tables, so
 that the two code paths share nothing after parsing.
 
-PLOG and Chebyshev reactions are not on the measured path and raise
-``NotImplementedError``.
+PLOG and Chebyshev rate forms follow rate_subs.py:446-485 and :138-223.
 """
 
 import math
@@ -148,6 +147,69 @@ def _kc_lines(rxn, specs):
     return out
 
 
+def _cheb_text(rxn):
+    """Chebyshev kf, rate_subs.py:138-223 (C: pow(10.0, ...))."""
+    t0, t1 = rxn.cheb_tlim
+    p0, p1 = rxn.cheb_plim
+    tsum = 1.0 / t0 + 1.0 / t1
+    tsub = 1.0 / t1 - 1.0 / t0
+    psum = math.log10(p0) + math.log10(p1)
+    psub = math.log10(p1) - math.log10(p0)
+    par = rxn.cheb_par
+    out = ['  Tred = ((2.0 / T) - {}) / {};\n'.format(fmt_e8(tsum), fmt_e8(tsub)),
+           '  Pred = (2.0 * log10(pres) - {}) / {};\n'.format(fmt_e8(psum),
+                                                             fmt_e8(psub)),
+           '  cheb_temp_0 = 1;\n', '  cheb_temp_1 = Pred;\n']
+    for i in range(rxn.cheb_n_temp):
+        out.append('  dot_prod[{}]= {} + Pred * {};\n'.format(
+            i, fmt_e8(par[i][0]), fmt_e8(par[i][1])))
+    one = True
+    for jj in range(2, rxn.cheb_n_pres):
+        new, old = (1, 0) if one else (0, 1)
+        out.append('  cheb_temp_{o} = 2 * Pred * cheb_temp_{n} - cheb_temp_{o};'
+                   '\n'.format(o=old, n=new))
+        for i in range(rxn.cheb_n_temp):
+            out.append('  dot_prod[{}] += {} * cheb_temp_{};\n'.format(
+                i, fmt_e8(par[i][jj]), old))
+        one = not one
+    out += ['  cheb_temp_0 = 1;\n', '  cheb_temp_1 = Tred;\n',
+            '  kf = dot_prod[0] + Tred * dot_prod[1];\n']
+    one = True
+    for i in range(2, rxn.cheb_n_temp):
+        new, old = (1, 0) if one else (0, 1)
+        out.append('  cheb_temp_{o} = 2 * Tred * cheb_temp_{n} - cheb_temp_{o};'
+                   '\n'.format(o=old, n=new))
+        out.append('  kf += dot_prod[{}] * cheb_temp_{};\n'.format(i, old))
+        one = not one
+    out.append('  kf = pow(10.0, kf);\n')
+    return ''.join(out)
+
+
+def _plog_text(rxn):
+    """PLOG kf, rate_subs.py:446-485: the bracketing pair interpolated
+    linearly in ln k over ln P, the end sets held constant outside."""
+    rows = rxn.plog_par
+    v = rows[0]
+    out = ['  if (pres <= {:.4e}) {{\n'.format(v[0]) +
+           '    kf = ' + arrhenius_text(v[1], v[2], v[3]) + ';\n']
+    for idx, v in enumerate(rows[:-1]):
+        v2 = rows[idx + 1]
+        out.append('  }} else if ((pres > {:.4e}) '.format(v[0]) +
+                   '&& (pres <= {:.4e})) {{\n'.format(v2[0]))
+        out.append('    kf = log(' + arrhenius_text(v[1], v[2], v[3]) + ');\n')
+        out.append('    kf2 = log(' + arrhenius_text(v2[1], v2[2], v2[3]) +
+                   ');\n')
+        out.append('    kf = exp(kf + (kf2 - kf) * (log(pres) - '
+                   '{}) / {});\n'.format(
+                       fmt_e16(math.log(v[0])),
+                       fmt_e16(math.log(v2[0]) - math.log(v[0]))))
+    v = rows[-1]
+    out.append('  }} else if (pres > {:.4e}) {{\n'.format(v[0]) +
+               '    kf = ' + arrhenius_text(v[1], v[2], v[3]) + ';\n' +
+               '  }\n')
+    return ''.join(out)
+
+
 def _write_rxn_rates(specs, reacs):
     rev_reacs = [i for i, r in enumerate(reacs) if r.rev]
     body = ['#include "rates.h"\n',
@@ -155,15 +217,31 @@ def _write_rxn_rates(specs, reacs):
             'const double * C, double * fwd_rxn_rates, '
             'double * rev_rxn_rates) {\n',
             '  double logT = log(T);\n', '\n']
+    kf_declared = False
     if rev_reacs and any(not r.rev_par for r in reacs):
         body.append('  double kf;\n  double Kc;\n')
+        kf_declared = True
+    if any(r.cheb for r in reacs):
+        if not kf_declared:
+            body.append('  double kf;\n')
+            kf_declared = True
+        body.append('  double Tred;\n  double Pred;\n')
+        body.append('  double cheb_temp_0, cheb_temp_1;\n')
+        body.append('  double dot_prod[{}];\n'.format(
+            max(r.cheb_n_temp for r in reacs if r.cheb)))
+    if any(r.plog for r in reacs):
+        if not kf_declared:
+            body.append('  double kf;\n')
+        body.append('  double kf2;\n')
     body.append('\n')
     for j, rxn in enumerate(reacs):
-        if rxn.cheb or rxn.plog:
-            raise NotImplementedError('PLOG/Chebyshev reactions')
         body.append('  //rxn {}\n'.format(j))
-        uses_kf = rxn.rev and not rxn.rev_par
-        if uses_kf:
+        uses_kf = (rxn.rev and not rxn.rev_par) or rxn.plog or rxn.cheb
+        if rxn.cheb:
+            body.append(_cheb_text(rxn))
+        elif rxn.plog:
+            body.append(_plog_text(rxn))
+        elif uses_kf:
             body.append('  kf = ' + arrhenius_text(rxn.A, rxn.b, rxn.E) +
                         ';\n')
         line = '  fwd_rxn_rates[{}] = '.format(j)

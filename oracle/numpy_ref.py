@@ -14,6 +14,12 @@ rather than from the reference's emitted text, used to pin the C oracle:
                    (Stewart, Larson & Golden, Combust. Flame 75, 1989)
 * falloff          pres_mod = F Pr/(1+Pr) (LOW given) or F/(1+Pr) (HIGH given)
 * species rates    wdot = nu^T q, a dense matrix product
+* PLOG             ln k linear in ln P between the bracketing table rows,
+                   the end rows held outside the table
+* Chebyshev        log10 k = sum_ij c_ij T_i(T~) T_j(P~) (numpy's chebval2d)
+                   with T~ = (2/T - 1/Tmin - 1/Tmax) / (1/Tmax - 1/Tmin),
+                   P~ = (2 log10 P - log10 Pmin - log10 Pmax) /
+                        (log10 Pmax - log10 Pmin)
 
 It reads the parsed data model (``SpecInfo``/``ReacInfo``), never the
 emitted C or the HIP tables, so that it shares nothing with either beyond
@@ -47,6 +53,30 @@ def _smh(specs, T):
 
 def _k(A, b, E, T):
     return A * T ** b * np.exp(-E / T)
+
+
+def _kf(r, T, P):
+    """Forward rate constant of reaction r at states (T, P), every form."""
+    if r.plog:
+        rows = r.plog_par
+        lk = [np.log(_k(A, b, E, T)) for _p, A, b, E in rows]
+        lp = [math.log(row[0]) for row in rows]
+        lP = np.log(P)
+        out = np.where(P <= rows[0][0], lk[0], lk[-1])
+        for i in range(len(rows) - 1):
+            inside = (P > rows[i][0]) & (P <= rows[i + 1][0])
+            val = lk[i] + (lk[i + 1] - lk[i]) * (lP - lp[i]) / (lp[i + 1] - lp[i])
+            out = np.where(inside, val, out)
+        return np.exp(out)
+    if r.cheb:
+        from numpy.polynomial import chebyshev
+        tmin, tmax = r.cheb_tlim
+        pmin, pmax = r.cheb_plim
+        tt = (2.0 / T - 1.0 / tmin - 1.0 / tmax) / (1.0 / tmax - 1.0 / tmin)
+        pp = ((2.0 * np.log10(P) - math.log10(pmin) - math.log10(pmax)) /
+              (math.log10(pmax) - math.log10(pmin)))
+        return 10.0 ** chebyshev.chebval2d(tt, pp, np.asarray(r.cheb_par))
+    return _k(r.A, r.b, r.E, T)
 
 
 def nu_matrix(specs, reacs):
@@ -83,7 +113,7 @@ def evaluate(specs, reacs, T, P, Y):
     pm = []
     q = np.empty((nr, T.size))
     for j, r in enumerate(reacs):
-        kf = _k(r.A, r.b, r.E, T)
+        kf = _kf(r, T, P)
         cf = np.ones_like(T)
         for k, v in zip(r.reac, r.reac_nu):
             cf = cf * C[k] ** v
@@ -194,9 +224,36 @@ def mp_state(specs, reacs, T, P, Y, dps=50):
     def k(A, b, E):
         return mp.mpf(A) * T ** mp.mpf(b) * mp.exp(-mp.mpf(E) / T)
 
+    def kf_of(r):
+        if r.plog:
+            rows = r.plog_par
+            if P <= rows[0][0]:
+                return k(*rows[0][1:4])
+            for i in range(len(rows) - 1):
+                if rows[i][0] < P <= rows[i + 1][0]:
+                    l0 = mp.log(k(*rows[i][1:4]))
+                    l1 = mp.log(k(*rows[i + 1][1:4]))
+                    w = ((mp.log(P) - mp.log(rows[i][0])) /
+                         (mp.log(rows[i + 1][0]) - mp.log(rows[i][0])))
+                    return mp.exp(l0 + (l1 - l0) * w)
+            return k(*rows[-1][1:4])
+        if r.cheb:
+            tmin, tmax = (mp.mpf(v) for v in r.cheb_tlim)
+            pmin, pmax = (mp.mpf(v) for v in r.cheb_plim)
+            tt = (2 / T - 1 / tmin - 1 / tmax) / (1 / tmax - 1 / tmin)
+            pp = ((2 * mp.log10(P) - mp.log10(pmin) - mp.log10(pmax)) /
+                  (mp.log10(pmax) - mp.log10(pmin)))
+            acc = mp.mpf(0)
+            for i in range(r.cheb_n_temp):
+                for j in range(r.cheb_n_pres):
+                    acc += (mp.mpf(float(r.cheb_par[i][j])) *
+                            mp.chebyt(i, tt) * mp.chebyt(j, pp))
+            return mp.power(10, acc)
+        return k(r.A, r.b, r.E)
+
     wdot = [mp.mpf(0)] * nsp
     for r in reacs:
-        kf = k(r.A, r.b, r.E)
+        kf = kf_of(r)
         f = kf
         for s, v in zip(r.reac, r.reac_nu):
             f *= C[s] ** v

@@ -18,7 +18,7 @@ from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-MECHS = ['mech13', 'mech53', 'mech111', 'mech13co']
+MECHS = ['mech13', 'mech53', 'mech111', 'mech13co', 'mech21pc']
 # tau of the conditioned error: where |wdot_k| < tau * S_k, errors count
 # relative to tau * S_k.  Every rate constant is exp(x) with |x| up to
 # ~100 (ln A, E/T and the NASA polynomial of ln Kc), and two correct FP64
@@ -144,6 +144,31 @@ def test_wdot_subset_matches_full_batch(mode):
     np.testing.assert_array_equal(Wh[:, sel], full[:, sel])
     rest = np.setdiff1d(np.arange(3000), sel)
     assert np.isnan(Wh[:, rest]).all()
+
+
+def test_pdep_forms_over_wide_pressures():
+    """PLOG (below, inside and above the table) and Chebyshev reactions:
+    stage rates pointwise and the fused kernels conditioned, against the
+    oracle at pressures from 0.005 to 500 atm."""
+    name = 'mech21pc'
+    specs, reacs, _ = mechanism(name)
+    T, P, Y = _states(name, 8192, 18)
+    rng = np.random.default_rng(19)
+    P = np.exp(rng.uniform(np.log(0.005 * 101325.0), np.log(500 * 101325.0),
+                           T.size))
+    ref = oracle(name).eval_stages(T, P, Y)
+    lib = hip_lib(name)
+    Td, Pd, Yd = to_dev(T, P, Y)
+    C = lib.eval_conc(Td, Pd, Yd)['C']
+    fwd, rev = lib.eval_rxn_rates(Td, Pd, C)
+    fe = np.abs(to_host(fwd) - ref['fwd']) / np.abs(ref['fwd'])
+    re_ = np.abs(to_host(rev) - ref['rev']) / np.maximum(np.abs(ref['rev']),
+                                                         1e-300)
+    S = metrics.gross_scale(specs, reacs, ref['fwd'], ref['rev'], ref['pm'])
+    W = to_host(lib.eval_wdot(Td, Pd, Yd))
+    we = metrics.rel_err(W, ref['wdot'], S, TAU).max()
+    print('fwd', fe.max(), 'rev', re_.max(), 'wdot', we)
+    assert fe.max() <= 1e-12 and re_.max() <= 1e-12 and we <= 1e-12
 
 
 def test_wdot_empty_and_bad_args():
