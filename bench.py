@@ -389,6 +389,8 @@ def main():
                                             args.cpu_sample, threads)
         log('cpu baseline: {:.3e} states/s'.format(
             line['cpu_baseline']['value']))
+    line['dydt'] = dydt_section(args, torch, lib, Td, Pd, Yd)
+    log('dydt: {:.3e} states/s'.format(line['dydt']['states_per_s']))
     del Td, Pd, Yd, Wd
 
     if not args.no_mixed and (lib.modes & 2):
@@ -425,8 +427,6 @@ def main():
         line['mixed'] = mixed
         log('mixed: {:.3e} states/s, speed-up {:.2f}'.format(
             mixed['states_per_s'], mixed['speedup_vs_fp64']))
-    line['dydt'] = dydt_section(args, torch, lib, Td, Pd, Yd)
-    log('dydt: {:.3e} states/s'.format(line['dydt']['states_per_s']))
     if not args.no_large:
         line['large'] = large_section(args, torch)
         log('large: {:.3e} states/s fp64'.format(

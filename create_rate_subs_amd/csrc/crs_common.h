@@ -110,6 +110,28 @@ __device__ __forceinline__ double exp64(double x)
     return __builtin_ldexp(p, (int)n);
 }
 
+// 2^t in FP64, same polynomial: t = n + s with n = rint(t) and s = t - n
+// exact, r = s ln2 (one rounding, |r| <= ln2/2), then as exp64.  For
+// exponents formed in base 2 (constants pre-scaled by log2 e), which saves
+// exp64's conversion multiply and its two-step reduction.
+__device__ __forceinline__ double exp2_64(double t)
+{
+    const double n = __builtin_rint(t);
+    const double r = (t - n) * 0.6931471805599453;
+    double q = 2.510038549551032e-08;
+    q = __builtin_fma(q, r, 2.7620088445409746e-07);
+    q = __builtin_fma(q, r, 2.7557268459997064e-06);
+    q = __builtin_fma(q, r, 2.4801521295954376e-05);
+    q = __builtin_fma(q, r, 0.00019841269863053618);
+    q = __builtin_fma(q, r, 0.0013888888917213717);
+    q = __builtin_fma(q, r, 0.008333333333330062);
+    q = __builtin_fma(q, r, 0.04166666666662413);
+    q = __builtin_fma(q, r, 0.16666666666666669);
+    q = __builtin_fma(q, r, 0.5000000000000001);
+    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
 // Per-lane LDS storage.  A copy of an LDS pointer taken through an empty asm
 // is a value the compiler cannot see through: loads made through it can be
 // neither hoisted above the point where the copy is taken nor forwarded from
@@ -123,6 +145,16 @@ __device__ __forceinline__ CRS_LDS double *opaque_lds(CRS_LDS double *p)
     return p;
 }
 
+// s/R - h/RT of one species from its 7 coefficients (in LDS) on the basis
+// [1, ln T, T, T^2, T^3, T^4, 1/T].
+__device__ __forceinline__ double nasa_smh(CRS_LDS const double *c,
+                                           double logT, double T, double T2,
+                                           double T3, double T4, double iT)
+{
+    return c[0] + c[1] * logT + c[2] * T + c[3] * T2 + c[4] * T3 +
+           c[5] * T4 + c[6] * iT;
+}
+
 // ---------------------------------------------------------------------------
 // Mixed-precision building blocks.  The argument of an exponential is formed
 // and range-reduced in FP64 (x log2 e = n + f, |f| <= 1/2, exact since
@@ -133,6 +165,15 @@ __device__ __forceinline__ CRS_LDS double *opaque_lds(CRS_LDS double *p)
 __device__ __forceinline__ double exp_mixed(double x)
 {
     const double t = x * 1.4426950408889634;   // log2(e)
+    const double n = __builtin_rint(t);
+    const float f = (float)(t - n);
+    return __builtin_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+// 2^t with the same split (t = n + f, only 2^f on v_exp_f32): the base-2
+// form of exp_mixed for exponents already scaled by log2 e.
+__device__ __forceinline__ double exp2_mixed(double t)
+{
     const double n = __builtin_rint(t);
     const float f = (float)(t - n);
     return __builtin_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);

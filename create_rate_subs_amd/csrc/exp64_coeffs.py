@@ -54,6 +54,36 @@ def emulate(x, cs, hi, lo):
     return rd(mp.ldexp(p, int(n)))
 
 
+def emulate2(t, cs, hi):
+    """crs::exp2_64: n = rint(t), r = (t - n) ln2 (t - n exact)."""
+    def rd(v):
+        return mp.mpf(float(v))
+
+    def fma(a, b, c):
+        return rd(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+    n = mp.nint(t)
+    r = rd((mp.mpf(t) - n) * hi)
+    q = mp.mpf(cs[0])
+    for c in cs[1:]:
+        q = fma(q, r, c)
+    p = fma(r, fma(r, q, 1.0), 1.0)
+    return rd(mp.ldexp(p, int(n)))
+
+
+def max_error2(samples=20000, seed=2):
+    """Max relative error of crs::exp2_64 against 2^t, units of 2^-53."""
+    cs = coefficients()
+    hi, _lo = ln2_split()
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        t = rng.uniform(-1000.0, 1000.0) if i % 2 else rng.uniform(-3.0, 3.0)
+        ex = mp.power(2, mp.mpf(t))
+        worst = max(worst, float(abs(emulate2(t, cs, hi) - ex) / ex))
+    return worst / 2.0 ** -53
+
+
 def max_error(samples=20000, seed=1):
     """Max relative error of the emulated sequence, in units of 2^-53."""
     cs = coefficients()
@@ -71,4 +101,5 @@ if __name__ == '__main__':
     print('q coefficients:', [repr(c) for c in coefficients()])
     print('ln2 split:', [repr(v) for v in ln2_split()])
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 20000
-    print('max error (units of 2^-53):', max_error(n))
+    print('exp64 max error (units of 2^-53):', max_error(n))
+    print('exp2_64 max error (units of 2^-53):', max_error2(n))

@@ -656,7 +656,12 @@ def _indent(lines, n):
 #   per-lane LDS rows removes the spills at 53 species (FP64 1.54 -> 1.37 ms,
 #   mixed 1.07 -> 0.88 ms with the H/G net-rate form below) and cuts them
 #   by 60 % at 111 species (FP64 7.2 -> 4.2 ms, mixed 7.9 -> 3.0 ms);
-# * the retiring reaction order and the branch-free exp64 add a few %.
+# * the retiring reaction order and the branch-free exp64 add a few %;
+# * base-2 Arrhenius exponents (exp2_64 / exp2_mixed) cut 2 % of the
+#   instructions, within timing noise;
+# * the NASA coefficients as a per-block LDS table selected per lane
+#   (nasa='lds') cut 4.5 % (FP64) / 8 % (mixed) of the instructions but
+#   run 3 % slower (LDS latency at one wave per SIMD): not the default.
 # 'auto' picks the H/G form with G in LDS when every species fits the LDS
 # budget (78 rows), otherwise species Kc with 78 concentration rows in LDS.
 DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
@@ -664,10 +669,18 @@ DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
 
 
 def resolve_opts(nsp, opts=None):
-    """Emitter options with 'auto' choices made for an NSP-species mechanism."""
+    """Emitter options with 'auto' choices made for an NSP-species mechanism.
+
+    LDS budget at one wave per SIMD: 160 KB per CU for 4 waves.  Per-lane
+    rows take 512 B per wave each; the NASA coefficient table (option
+    nasa='lds') takes 128 B per species per block.
+    """
     o = dict(DEFAULT_OPTS, **(opts or {}))
+    if 'lds_species' not in o:
+        table = 128 * nsp if o.get('nasa') == 'lds' else 0
+        o['lds_species'] = min(78, (160 * 1024 - table - 1024) // 2048)
     if o.get('kc') == 'auto':
-        if nsp <= o.get('lds_species', 78):
+        if nsp <= o['lds_species']:
             o['kc'] = 'species_hg'
             o.setdefault('g_store', 'lane_lds')
         else:
@@ -765,23 +778,32 @@ def _fused_f64_lines(ir, K, opts):
 _BASIS = ('1.0', 'logT', 'T', 'T2', 'T3', 'T4', 'iT')
 
 
-def _fast_arr(a, K, name, expfn='exp'):
+_LOG2E = 1.4426950408889634
+
+
+def _fast_arr(a, K, name, expfn='exp', log2=False):
     """Lines defining rate constant ``name`` (and its exponent ``x_<name>``).
 
     Returns (lines, exponent_or_None).  The exponent is lnA + b logT - E/T
     with 1/T precomputed once per state; forms without an exponential
-    (literal A, A*T^n) are kept as the reference prints them.
+    (literal A, A*T^n) are kept as the reference prints them.  With
+    ``log2`` the exponent is formed in base 2 (log2 A + b log2 T -
+    (E log2 e) / T, constants scaled once here) for an exponential that
+    takes a base-2 argument, saving its conversion multiply.
     """
     if not a.has_exp:
         return ['const double {} = {};'.format(name, _arr_expr(a, K))], None
+    sc = _LOG2E if log2 else 1.0
+    lt = 'log2T' if log2 else 'logT'
     terms = []
     if a.logA is not None:
-        terms.append(K(a.logA))
+        terms.append(K(a.logA * sc))
     if a.b is not None:
-        terms.append('{} * logT'.format(K(a.b)))
+        terms.append('{} * {}'.format(K(a.b), lt))
     x = ' + '.join(terms)
     if a.E is not None:
-        x = '{} - {} * iT'.format(x, K(a.E)) if x else '-{} * iT'.format(K(a.E))
+        x = ('{} - {} * iT'.format(x, K(a.E * sc)) if x else
+             '-{} * iT'.format(K(a.E * sc)))
     out = ['const double x_{} = {};'.format(name, x)]
     if a.pre is not None:
         out.append('const double {n} = {p} * {e}(x_{n});'.format(
@@ -805,8 +827,11 @@ def _fast_lnkc(x, K):
     return ' + '.join(terms) if terms else '0.0'
 
 
-def _fast_pm_lines(x, K, kf_x):
-    """Pressure modification of reaction x into 'pm' (fast FP64 form)."""
+def _fast_pm_lines(x, K, kf_x, arr2=None):
+    """Pressure modification of reaction x into 'pm' (fast FP64 form).
+
+    With ``arr2`` (a base-2 exponential) the Arrhenius exponents, kf's
+    included, are base 2."""
     out = []
     if x.thd:
         out.append('const double pm = {};'.format(_eff_expr(x, K)))
@@ -818,20 +843,22 @@ def _fast_pm_lines(x, K, kf_x):
         coll = 'C[{}]'.format(x.pdep_sp)
     # k0 / kinf: whichever limit is not given is the main Arrhenius (kf)
     if x.low is not None:
-        l0, x0 = _fast_arr(x.low, K, 'k0')
+        l0, x0 = (_fast_arr(x.low, K, 'k0', arr2, log2=True) if arr2 else
+                  _fast_arr(x.low, K, 'k0'))
         out += l0
         num, xnum = 'k0', x0
     else:
         num, xnum = 'kf', kf_x
     if x.high is not None:
-        l1, x1 = _fast_arr(x.high, K, 'kinf')
+        l1, x1 = (_fast_arr(x.high, K, 'kinf', arr2, log2=True) if arr2 else
+                  _fast_arr(x.high, K, 'kinf'))
         out += l1
         den, xden = 'kinf', x1
     else:
         den, xden = 'kf', kf_x
     if xnum is not None and xden is not None:
-        out.append('const double Pr = {} * exp({} - {});'.format(coll, xnum,
-                                                                  xden))
+        out.append('const double Pr = {} * {}({} - {});'.format(
+            coll, arr2 or 'exp', xnum, xden))
     else:
         out.append('const double Pr = {} * {} / {};'.format(num, coll, den))
     tail = ''
@@ -920,8 +947,19 @@ def _gibbs_species(ir):
     return sorted(need)
 
 
+def _nasa_table_values(ir):
+    """Per species, per range (lower, upper): the 7 coefficients of
+    s/R - h/RT on [1, ln T, T, T^2, T^3, T^4, 1/T], padded to 8."""
+    vals = []
+    for _tm, lo, hi in ir.nasa:
+        for a in (lo, hi):
+            vals += [a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0, a[3] / 12.0,
+                     a[4] / 20.0, -a[5], 0.0]
+    return vals
+
+
 def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
-                         every=False, expfn='exp'):
+                         every=False, expfn='exp', nasa_lds=False):
     """Per-state G_k = exp(S_k/R - H_k/RT) and its inverse, once per species.
 
     With them Kc = prod_k G_k^nu_k (PA / (RU T))^dnu for every reversible
@@ -965,14 +1003,21 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
     for k in sorted(need_g | need_gi):
         sp = ir.nasa[k]
         gi = tmids.index(sp[0]) if sp[0] in tmids else None
-        terms = []
-        for half, a in (('l', sp[1]), ('h', sp[2])):
-            coef = [a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0, a[3] / 12.0,
-                    a[4] / 20.0, -a[5]]
-            for bi, cv in enumerate(coef):
-                if cv != 0.0:
-                    terms.append('{} * p{}{}_{}'.format(K(cv), half, gi, bi))
-        out.append('const double smh{} = {};'.format(k, ' + '.join(terms)))
+        if nasa_lds:
+            # this lane's range of the coefficient table in LDS
+            out.append('const double smh{k} = crs::nasa_smh(crs::opaque_lds('
+                       '(CRS_LDS double *)(crs_nasa_lds + {o} + nsel{g})), '
+                       'logT, T, T2, T3, T4, iT);'.format(k=k, o=16 * k, g=gi))
+        else:
+            terms = []
+            for half, a in (('l', sp[1]), ('h', sp[2])):
+                coef = [a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0,
+                        a[3] / 12.0, a[4] / 20.0, -a[5]]
+                for bi, cv in enumerate(coef):
+                    if cv != 0.0:
+                        terms.append('{} * p{}{}_{}'.format(K(cv), half, gi,
+                                                             bi))
+            out.append('const double smh{} = {};'.format(k, ' + '.join(terms)))
         if lds:
             out.append('sG[{s}][lx] = exp(smh{k});'.format(s=slots[k], k=k))
             continue
@@ -1051,22 +1096,23 @@ def _mixed_pm_lines(x, K, kf_x):
         out.append('const float Mf = {};'.format(_eff_expr_f(x)))
     else:
         out.append('const float Mf = (float)C[{}];'.format(x.pdep_sp))
+    # every exponent here is base 2 (the mixed kernel's Arrhenius form)
     if x.low is not None:
-        l0, x0 = _fast_arr(x.low, K, 'k0', 'crs::exp_mixed')
+        l0, x0 = _fast_arr(x.low, K, 'k0', 'crs::exp2_mixed', log2=True)
         out += l0
         num, xnum = 'k0', x0
     else:
         num, xnum = 'kf', kf_x
     if x.high is not None:
-        l1, x1 = _fast_arr(x.high, K, 'kinf', 'crs::exp_mixed')
+        l1, x1 = _fast_arr(x.high, K, 'kinf', 'crs::exp2_mixed', log2=True)
         out += l1
         den, xden = 'kinf', x1
     else:
         den, xden = 'kf', kf_x
     if xnum is not None and xden is not None:
         out.append('const double xd = {} - {};'.format(xnum, xden))
-        out.append('const double Pr = (double)Mf * crs::exp_mixed(xd);')
-        lpr = ('crs::log10f_hw(Mf) + (float)(xd * 0.43429448190325182)')
+        out.append('const double Pr = (double)Mf * crs::exp2_mixed(xd);')
+        lpr = ('crs::log10f_hw(Mf) + (float)(xd * 0.30102999566398120)')
     else:
         out.append('const double Pr = {} * (double)Mf / {};'.format(num, den))
         lpr = '(float)log10(fmax(Pr, 1.0e-300))'
@@ -1356,11 +1402,17 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
         lines += ['const double logP = log(P);',
                   'const double log10P = logP * 0.43429448190325182;',
                   '(void)log10P;']
+    lines += ['const double log2T = logT * 1.4426950408889634;',
+              '(void)log2T;']
     if mixed:
         lines += ['const float mf = (float)m;', 'const float Tf = (float)T;',
                   '(void)mf; (void)Tf;']
     fexp = 'crs::exp64' if opts.get('exp') == 'crs' else 'exp'
     expfn = 'crs::exp_mixed' if mixed else fexp
+    # Arrhenius exponentials in base 2 (constants pre-scaled by log2 e):
+    # exp2_mixed in the mixed kernel, exp2_64 in the FP64 one
+    arr2 = ('crs::exp2_mixed' if mixed else
+            'crs::exp2_64' if opts.get('exp') == 'crs' else None)
     tmids = []
     for x in ir.rxns:
         for tm, _lo, _hi in x.kc_groups:
@@ -1384,12 +1436,16 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
         lines += ['const double irho = 1.0 / rho;', 'double esum = 0.0;']
     hg = opts.get('kc') == 'species_hg'
     species_kc = opts.get('kc') in ('species', 'species_hg')
+    nasa_lds = opts.get('nasa') == 'lds'
+    if nasa_lds:
+        lines += ['const int nsel{g} = lo{g} ? 0 : 8;'.format(g=gi)
+                  for gi in range(len(tmids))]
     if species_kc:
         lines += _species_gibbs_lines(ir, K, tmids,
                                       opts.get('kc_inverse') == 'store',
                                       {'lds': True, 'lds_sync': 'sync'}.get(
                                           opts.get('g_store'), False),
-                                      every=hg, expfn=fexp)
+                                      every=hg, expfn=fexp, nasa_lds=nasa_lds)
     gslots = {k: s for s, k in enumerate(_gibbs_species(ir))}
     gname = ((lambda k: 'sG[{}][lx]'.format(gslots[k]))
              if opts.get('g_store') in ('lds', 'lds_sync') else None)
@@ -1437,13 +1493,15 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
             inner, xf = _plog_fast_lines(x, K, expfn), None
         elif x.cheb:
             inner, xf = _cheb_fast_lines(x, K, expfn), None
+        elif arr2:
+            inner, xf = _fast_arr(x.arr, K, 'kf', arr2, log2=True)
         else:
             inner, xf = _fast_arr(x.arr, K, 'kf', expfn)
         if hg and _hg_ok(x):
             pm = ''
             if x.pd_index is not None:
                 inner += (_mixed_pm_lines(x, K, xf) if mixed else
-                          _fast_pm_lines(x, K, xf))
+                          _fast_pm_lines(x, K, xf, arr2))
                 pm = ' * pm'
             inner += _hg_net_rate(x, gref, pm)
             if mfma:
@@ -1498,6 +1556,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
                     # Kc stays FP64; only the Arrhenius factor is mixed
                     inner.append('const double r = {}(kf * exp(-lnKc));'.format(
                         _conc_prod(x.prod)))
+                elif xf is not None and arr2:
+                    inner.append('const double r = {}{}({} - lnKc * '
+                                 '1.4426950408889634);'.format(
+                                     _conc_prod(x.prod), arr2, xf))
                 elif xf is not None:
                     inner.append('const double r = {}exp({} - lnKc);'.format(
                         _conc_prod(x.prod), xf))
@@ -1507,7 +1569,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
         q = '(f - r)' if x.rev else 'f'
         if x.pd_index is not None:
             inner += (_mixed_pm_lines(x, K, xf) if mixed else
-                      _fast_pm_lines(x, K, xf))
+                      _fast_pm_lines(x, K, xf, arr2))
             q = q + ' * pm'
         inner.append('const double q = {};'.format(q))
         if mfma:
@@ -1683,6 +1745,19 @@ def emit_hip_source(ir, opts=None):
         '    double *__restrict__ dy_N, int64_t n, int64_t ld)\n{\n'
         '#pragma clang fp contract(off)\n' + _indent(lines, 4) + '}\n\n')
 
+    # ---- NASA coefficient table in LDS (option nasa='lds') -------------
+    nasa_fill = ''
+    if opts.get('nasa') == 'lds':
+        vals = _nasa_table_values(ir)
+        body.append('__device__ const double crs_nasa_tab[{}] = {{\n'.format(
+            len(vals)) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
+                                               else ' ')
+                                 for q, v in enumerate(vals)) + '};\n')
+        body.append('__shared__ double crs_nasa_lds[{}];\n\n'.format(len(vals)))
+        nasa_fill = ('    for (int q = threadIdx.x; q < {n}; q += blockDim.x)\n'
+                     '        crs_nasa_lds[q] = crs_nasa_tab[q];\n'
+                     '    __syncthreads();\n'.format(n=len(vals)))
+
     # ---- fused FP64 per-state body ------------------------------------
     body.append(
         '__device__ __forceinline__ void wdot_f64_state(\n'
@@ -1698,7 +1773,8 @@ def emit_hip_source(ir, opts=None):
         '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
-        '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
+        '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
+        nasa_fill +
         '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
         '    if (t >= m) return;\n'
         '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
@@ -1717,7 +1793,7 @@ def emit_hip_source(ir, opts=None):
         '__global__ __launch_bounds__(crs::kBlock) void k_dydt_f64(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
-        '    int64_t ld)\n{\n'
+        '    int64_t ld)\n{\n' + nasa_fill +
         '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
         '    if (i >= n) return;\n'
         '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
@@ -1745,6 +1821,7 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
             '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
+            + nasa_fill +
             '    // whole waves beyond m leave; within a wave every lane runs (the\n'
             '    // contraction exchanges data across lanes), dead lanes on state 0\n'
             '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
@@ -1767,7 +1844,8 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
-            '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
+            '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
+            nasa_fill +
             '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
             '    if (t >= m) return;\n'
             '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'

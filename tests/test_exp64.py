@@ -22,9 +22,9 @@ def _coeffs_module():
     return mod
 
 
-def _device_constants():
+def _device_constants(fn='double exp64(double x)'):
     text = open(os.path.join(CSRC, 'crs_common.h')).read()
-    body = text[text.index('double exp64(double x)'):]
+    body = text[text.index(fn):]
     body = body[:body.index('\n}\n')]
     return [float(v) for v in re.findall(r'(-?\d+\.\d+(?:e-?\d+)?)', body)]
 
@@ -41,3 +41,12 @@ def test_header_holds_derived_coefficients():
 def test_emulated_sequence_error_bound():
     mod = _coeffs_module()
     assert mod.max_error(samples=3000, seed=7) <= 1.5
+
+
+def test_exp2_64_shares_the_polynomial_and_error_bound():
+    mod = _coeffs_module()
+    consts = _device_constants('double exp2_64(double t)')
+    hi, _lo = mod.ln2_split()
+    assert consts[0] == hi
+    assert consts[1:11] == mod.coefficients()
+    assert mod.max_error2(samples=3000, seed=8) <= 1.5

@@ -171,6 +171,49 @@ def test_pdep_forms_over_wide_pressures():
     assert fe.max() <= 1e-12 and re_.max() <= 1e-12 and we <= 1e-12
 
 
+def test_full_size_properties():
+    """At the headline mixed-precision size (53 sp, 2^22 states), where the
+    oracle would take minutes: the fused FP64 kernel against the chained
+    stage kernels (a separate code path, pinned against the oracle above)
+    to 1e-12 conditioned, and element and mass conservation of the fused
+    rates to 1e-12 of their gross scale."""
+    import torch
+    name = 'mech53'
+    specs, reacs, _ = mechanism(name)
+    n = 1 << 22
+    T, P, Y = _states(name, n, 20)
+    lib = hip_lib(name)
+    Td, Pd, Yd = to_dev(T, P, Y)
+    del T, P, Y
+    W = lib.eval_wdot(Td, Pd, Yd)
+    Wst = torch.empty_like(W)
+    S = torch.empty_like(W)
+    chunk = 1 << 20
+    for a in range(0, n, chunk):
+        b = a + chunk
+        Tc, Pc = Td[a:b].contiguous(), Pd[a:b].contiguous()
+        C = lib.eval_conc(Tc, Pc, Yd[:, a:b].contiguous())['C']
+        fwd, rev = lib.eval_rxn_rates(Tc, Pc, C)
+        pm = lib.get_rxn_pres_mod(Tc, Pc, C)
+        Wst[:, a:b] = lib.eval_spec_rates(fwd, rev, pm)
+        S[:, a:b] = metrics.gross_scale(specs, reacs, fwd, rev, pm)
+        del C, fwd, rev, pm
+    e = metrics.rel_err(W, Wst, S, TAU).max().item()
+    mw = torch.tensor([sp.mw for sp in specs], dtype=torch.float64,
+                      device='cuda')
+    elems = sorted({el.upper() for sp in specs for el, _c in sp.elem})
+    A = torch.zeros((len(elems) + 1, len(specs)), dtype=torch.float64,
+                    device='cuda')
+    for k, sp in enumerate(specs):
+        for el, c in sp.elem:
+            A[elems.index(el.upper()), k] += c
+    A[-1] = mw
+    resid = (A @ W).abs() / (A.abs() @ S).clamp_min(1e-300)
+    print('fused vs stages', e, 'balance residual', resid.max().item())
+    assert e <= 1e-12
+    assert resid.max().item() <= 1e-12
+
+
 def test_wdot_empty_and_bad_args():
     import ctypes
     import torch
