@@ -1798,6 +1798,44 @@ def emit_hip_source(ir, opts=None):
         '    if (i >= n) return;\n'
         '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
 
+    # ---- RK4 stage combination for the batched integrator ------------
+    body.append(
+        '// One RK4 stage on the state rows 0..NSP-1 (row 0 = T), written\n'
+        '// without contraction so that it is the textbook sequence exactly:\n'
+        '//   1: acc = k,      ytmp = y + h/2 k\n'
+        '//   2: acc += 2 k,   ytmp = y + h/2 k\n'
+        '//   3: acc += 2 k,   ytmp = y + h k\n'
+        '//   4: y += h/6 (acc + k); a state whose T first reaches\n'
+        '//      T0 + dT_ign gets t_ign = t_new (when t_ign is given)\n'
+        '__global__ __launch_bounds__(crs::kBlock) void k_rk4_combine(\n'
+        '    double *__restrict__ y, double *__restrict__ ytmp,\n'
+        '    double *__restrict__ acc, const double *__restrict__ k,\n'
+        '    int64_t n, int64_t ld, int stage, double h, double t_new,\n'
+        '    const double *__restrict__ T0, double *__restrict__ t_ign,\n'
+        '    double dT_ign)\n{\n'
+        '#pragma clang fp contract(off)\n'
+        '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
+        '    if (i >= n) return;\n'
+        '    for (int r = 0; r < NSP; ++r) {\n'
+        '        const int64_t o = (int64_t)r * ld + i;\n'
+        '        const double kv = k[o];\n'
+        '        if (stage == 1) {\n'
+        '            acc[o] = kv;\n'
+        '            ytmp[o] = y[o] + 0.5 * h * kv;\n'
+        '        } else if (stage == 2) {\n'
+        '            acc[o] += 2.0 * kv;\n'
+        '            ytmp[o] = y[o] + 0.5 * h * kv;\n'
+        '        } else if (stage == 3) {\n'
+        '            acc[o] += 2.0 * kv;\n'
+        '            ytmp[o] = y[o] + h * kv;\n'
+        '        } else {\n'
+        '            y[o] += (h / 6.0) * (acc[o] + kv);\n'
+        '        }\n'
+        '    }\n'
+        '    if (stage == 4 && t_ign != nullptr && !(t_ign[i] >= 0.0) &&\n'
+        '        y[i] >= T0[i] + dT_ign)\n'
+        '        t_ign[i] = t_new;\n}\n\n')
+
     # ---- fused mixed-precision per-state body -------------------------
     if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
         rx = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
@@ -2050,6 +2088,42 @@ int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
     k_dydt_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, P, y, dy, n, ld);
+    return crs::launched(fn);
+}}
+
+int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
+            double t0, double h, int nsteps, const double *T0, double *t_ign,
+            double dT_ign, void *stream)
+{{
+    const char *fn = "crs_rk4";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (nsteps < 0)
+        return crs::fail(CRS_ERR_ARG, "%s: nsteps must be >= 0", fn);
+    if (!(h > 0.0))
+        return crs::fail(CRS_ERR_ARG, "%s: h must be > 0", fn);
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, y, "y", n));
+    CRS_TRY(crs::check_ptr(fn, work, "work", n));
+    if ((T0 == nullptr) != (t_ign == nullptr))
+        return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
+    if (n == 0 || nsteps == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    double *ytmp = work;
+    double *acc = work + (int64_t)NSP * ld;
+    double *k = work + 2 * (int64_t)NSP * ld;
+    const dim3 grid = crs::grid_for(n);
+    double t = t0;
+    for (int step = 0; step < nsteps; ++step) {{
+        for (int stage = 1; stage <= 4; ++stage) {{
+            k_dydt_f64<<<grid, crs::kBlock, 0, s>>>(K, P, stage == 1 ? y : ytmp,
+                                                    k, n, ld);
+            k_rk4_combine<<<grid, crs::kBlock, 0, s>>>(
+                y, ytmp, acc, k, n, ld, stage, h, t + h, T0, t_ign, dT_ign);
+        }}
+        t += h;
+    }}
     return crs::launched(fn);
 }}
 

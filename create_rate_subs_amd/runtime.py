@@ -23,7 +23,7 @@ MODES = {'fp64': 0, 'mixed': 1}
 EXPORTED_SYMBOLS = ('crs_get_mech_info', 'crs_last_error', 'crs_eval_conc',
                     'crs_eval_rxn_rates', 'crs_get_rxn_pres_mod',
                     'crs_eval_spec_rates', 'crs_eval_wdot',
-                    'crs_eval_wdot_subset', 'crs_dydt')
+                    'crs_eval_wdot_subset', 'crs_dydt', 'crs_rk4')
 
 
 class CrsError(RuntimeError):
@@ -49,6 +49,8 @@ _SIGS = {
     'crs_eval_wdot': [_P] * 4 + [_I64, _I64, ctypes.c_int, _P],
     'crs_eval_wdot_subset': [_P] * 5 + [_I64, _I64, ctypes.c_int, _P],
     'crs_dydt': [_P] * 3 + [_I64, _I64, ctypes.c_int, _P],
+    'crs_rk4': [_P] * 3 + [_I64, _I64, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_int, _P, _P, ctypes.c_double, _P],
 }
 
 
@@ -193,6 +195,31 @@ class MechLib(object):
         self._check(self.lib.crs_dydt(_ptr(P), _ptr(y), _ptr(dy), n, ld,
                                       MODES[mode], _stream()))
         return dy
+
+    def rk4(self, P, y, t0, h, nsteps, work=None, T0=None, t_ign=None,
+            dT_ign=400.0):
+        """nsteps RK4 steps of the CONP system on y (updated in place);
+        returns y.  ``work``: [3 NSP][ld] scratch (allocated if None)."""
+        n = P.numel()
+        self._vec(P, n, 'P')
+        rows = y.shape[0] if y.dim() == 2 else -1
+        if rows not in (self.nsp, self.nsp + 1):
+            raise ValueError('y must have NSP or NSP+1 rows')
+        ld = self._rows(y, rows, n, 'y')
+        if work is None:
+            work = self._new(3 * self.nsp, n, ld)
+        elif self._rows(work, 3 * self.nsp, n, 'work') != ld:
+            raise ValueError('work must share the row stride of y')
+        if (T0 is None) != (t_ign is None):
+            raise ValueError('T0 and t_ign go together')
+        if T0 is not None:
+            self._vec(T0, n, 'T0')
+            self._vec(t_ign, n, 't_ign')
+        self._check(self.lib.crs_rk4(_ptr(P), _ptr(y), _ptr(work), n, ld,
+                                     float(t0), float(h), int(nsteps),
+                                     _ptr(T0), _ptr(t_ign), float(dT_ign),
+                                     _stream()))
+        return y
 
     def eval_wdot_subset(self, T, P, Y, W, idx, mode='fp64'):
         n = T.numel()

@@ -137,6 +137,24 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
 int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
              int64_t ld, int mode, void *stream);
 
+/*
+ * Device-resident batched integrator: nsteps classical RK4 steps of size h
+ * (seconds) of the constant-pressure system crs_dydt defines, starting at
+ * time t0, updating y [rows][ld] in place.  Replaces the time loop of the
+ * reference's generated main (create_rate_subs.py:210-283: intDriver
+ * per step, with the state copied host <-> device around every step and
+ * the integrator left as an INTEGRATOR macro): here the state never leaves
+ * the device and the step is RK4.  work: 3 * NSP rows of stride ld
+ * (caller-owned scratch).  Optional ignition bookkeeping as the reference's
+ * IGN block (:259-266): with T0[n] and t_ign[n] (initialise t_ign < 0), a
+ * state whose T first reaches T0 + dT_ign gets t_ign = the end time of that
+ * step.  Enqueues 8 kernels per step on the stream (capturable in a graph;
+ * no memset, no allocation).
+ */
+int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
+            double t0, double h, int nsteps, const double *T0, double *t_ign,
+            double dT_ign, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

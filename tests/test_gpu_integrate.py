@@ -1,0 +1,136 @@
+"""Batched device-resident integrator (crs_rk4, integrate.py) on the GPU.
+
+The synthetic mechanisms are extremely stiff (species time scales down to
+1e-57 s: random thermochemistry makes some reverse rates enormous), so no
+explicit step of physical size stays finite; these tests check the
+integrator's own arithmetic, which is what it adds to crs_dydt (pinned
+against the oracle elsewhere), at steps small enough to stay finite:
+
+* RK4 on the GPU is bit-identical to the textbook RK4 combination done on
+  the host with the stage derivatives the GPU's crs_dydt returns for the
+  host's own stage states (the combination kernel is not contracted);
+* the harness' captured HIP graph, replayed several times from the same
+  initial state with eager work and synchronisations in between, gives the
+  eager result every time;
+* ignition bookkeeping (the reference's IGN block) flags a state the step
+  its temperature first reaches T0 + dT_ign, and no other;
+* the step count follows the reference's while (t < tend) loop (floating
+  accumulation included).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from create_rate_subs_amd import integrate, states
+
+from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+# near-equilibrium states' fastest species move by up to ~1e-3 of their
+# value per step at these sizes, and every stage stays finite
+H_OF = {'mech13co': 1.0e-39, 'mech53': 1.0e-39, 'mech21pc': 1.0e-30}
+H = H_OF['mech53']
+
+
+def _state(name, n, seed):
+    specs, _r, _f = mechanism(name)
+    nsp = len(specs)
+    T, P, Y = states.equilibrium_states(specs, n, seed, perturb=1e-3)
+    y = np.empty((nsp, n))
+    y[0] = T
+    y[1:] = Y[:nsp - 1]
+    return P, y
+
+
+def _rk4_host(lib, Pd, y, h, steps):
+    """Textbook RK4 on the host, stage derivatives from the GPU's dydt."""
+    def f(v):
+        return to_host(lib.dydt(Pd, to_dev(v)[0]))
+    y = y.copy()
+    for _ in range(steps):
+        k1 = f(y)
+        k2 = f(y + 0.5 * h * k1)
+        k3 = f(y + 0.5 * h * k2)
+        k4 = f(y + h * k3)
+        y = y + (h / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    return y
+
+
+@pytest.mark.parametrize('name', ['mech13co', 'mech53', 'mech21pc'])
+def test_rk4_is_the_textbook_combination(name):
+    P, y0 = _state(name, 256, 61)
+    h, steps = H_OF[name], 3
+    lib = hip_lib(name)
+    Pd, yd = to_dev(P, y0)
+    ref = _rk4_host(lib, Pd, y0, h, steps)
+    lib.rk4(Pd, yd, 0.0, h, steps)
+    got = to_host(yd)
+    moved = (got != y0).mean()
+    print(name, 'entries changed by the steps', moved)
+    assert np.isfinite(got).all()
+    assert moved > 0.01
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_graph_replays_match_eager():
+    name = 'mech53'
+    P, y0 = _state(name, 1000, 62)
+    lib = hip_lib(name)
+    Pd, y0d = to_dev(P, y0)
+    h, steps = H, 3
+    eager = y0d.clone()
+    lib.rk4(Pd, eager, 0.0, h, steps)
+    y = y0d.clone()
+    work = torch.empty((3 * lib.nsp, 1000), dtype=torch.float64,
+                       device='cuda')
+    lib.rk4(Pd, y, 0.0, h, 1, work=work)      # set-up outside the capture
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        lib.rk4(Pd, y, 0.0, h, steps, work=work)
+    for rep in range(3):
+        y.copy_(y0d)
+        other = lib.eval_wdot(Pd, Pd, torch.ones((lib.nsp, 1000),
+                                                 dtype=torch.float64,
+                                                 device='cuda'))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, eager), rep
+        del other
+
+
+def test_ignition_time_is_the_first_step_reaching_threshold():
+    name = 'mech53'
+    P, y0 = _state(name, 64, 63)
+    lib = hip_lib(name)
+    Pd, yd = to_dev(P, y0)
+    h = H
+    # threshold already met for the first half, unreachable for the rest
+    T0 = torch.tensor(np.where(np.arange(64) < 32, y0[0] - 1000.0,
+                               y0[0] + 1.0e6), device='cuda')
+    t_ign = torch.full((64,), -1.0, dtype=torch.float64, device='cuda')
+    lib.rk4(Pd, yd, 0.0, h, 3, T0=T0, t_ign=t_ign, dT_ign=400.0)
+    ti = to_host(t_ign)
+    assert (ti[:32] == h).all()
+    assert (ti[32:] == -1.0).all()
+
+
+def test_harness_runs_the_reference_loop_on_device():
+    name = 'mech53'
+    P, y0 = _state(name, 4096, 64)
+    lib = hip_lib(name)
+    Pd, yd = to_dev(P, y0)
+    # the reference's defaults: t accumulates 1e-8 ten times to just below
+    # 1e-7, so its loop runs 11 steps (its timer divides by (tend-t0)/h = 10)
+    assert integrate.loop_steps(0.0, 1.0e-7, 1.0e-8) == 11
+    h = H
+    out = integrate.integrate(lib, Pd, yd, 0.0, 5.0 * H, h)
+    steps = integrate.loop_steps(0.0, 5.0 * H, h)
+    assert out['steps'] == steps
+    ref = to_dev(y0)[0]
+    lib.rk4(Pd, ref, 0.0, h, steps)
+    assert torch.equal(yd, ref)
+    print('steps', steps, 'ms/step', out['ms_per_step'])
