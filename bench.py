@@ -199,6 +199,26 @@ def dydt_section(args, torch, lib, T, P, Y):
                 kernel='k_dydt_f64')
 
 
+def integrate_section(args, torch, lib, T, P, Y, steps=10):
+    """The device-resident time loop (create_rate_subs_amd.integrate): RK4
+    steps of the CONP system on the headline states, the whole loop
+    replayed from one captured HIP graph.  The step is tiny (1e-39 s) so
+    that the synthetic mechanism's extreme stiffness cannot push values to
+    inf; the work per step does not depend on it."""
+    from create_rate_subs_amd import integrate
+    n = T.numel()
+    nsp = lib.nsp
+    y = torch.empty((nsp, n), dtype=torch.float64, device='cuda')
+    y[0] = T
+    y[1:] = Y[:nsp - 1]
+    h = 1.0e-39
+    out = integrate.integrate(lib, P, y, 0.0, steps * h, h)
+    return dict(workload='w1_53 states, {} RK4 steps (4 dydt + 4 stage '
+                         'kernels each) from one graph'.format(out['steps']),
+                steps=out['steps'], ms_per_step=out['ms_per_step'],
+                state_steps_per_s=n * out['steps'] / (out['ms_total'] * 1e-3))
+
+
 def _quantiles(torch, x, qs=(0.01, 0.1, 0.5, 0.9, 0.99)):
     """Quantiles of a per-state error vector (strided sample if large)."""
     x = x[::max(1, x.numel() >> 20)].double()
@@ -391,6 +411,9 @@ def main():
             line['cpu_baseline']['value']))
     line['dydt'] = dydt_section(args, torch, lib, Td, Pd, Yd)
     log('dydt: {:.3e} states/s'.format(line['dydt']['states_per_s']))
+    line['integrate'] = integrate_section(args, torch, lib, Td, Pd, Yd)
+    log('integrate: {:.3f} ms per RK4 step'.format(
+        line['integrate']['ms_per_step']))
     del Td, Pd, Yd, Wd
 
     if not args.no_mixed and (lib.modes & 2):
