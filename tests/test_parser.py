@@ -202,3 +202,20 @@ def test_calc_spec_smh_closed_form(files):
              a[4] * T ** 4 / 5 + a[5] / T)
         got = chem.calc_spec_smh(T, [sp])[0]
         assert abs(got - (s - h)) <= 1e-12 * max(1.0, abs(s - h))
+
+
+def test_species_names_ending_in_plus(tmp_path):
+    """Ion names ('HCO+', 'H3O+'): mech_interpret.py:277-285 splits a side
+    on '+' without removing blanks and glues an EMPTY piece onto the one
+    before it, so 'HCO++H2O' reads as HCO+ and H2O, while 'HCO+ + H2O'
+    leaves a blank piece (an unknown species '' there and here alike).
+    Names holding '(+)' are re-joined (:287-296)."""
+    sides = [mi._split_side(s)[0] for s in ('HCO++H2O', 'H3O++CO')]
+    assert [t[0].strip() for t in sides[0]] == ['HCO+', 'H2O']
+    assert [t[0].strip() for t in sides[1]] == ['H3O+', 'CO']
+    blank = mi._split_side('HCO+ + H2O')[0]
+    assert [t[0].strip() for t in blank] == ['HCO', '', 'H2O']
+    assert [t[0] for t in mi._split_side('A(+)B+C')[0]] == ['A(+)B', 'C']
+    terms, pdep, pdep_sp, thd = mi._split_side('CH2(S) + O2 (+AR)')
+    assert [t[0].strip() for t in terms] == ['CH2(S)', 'O2']
+    assert pdep and pdep_sp == 'AR' and not thd
