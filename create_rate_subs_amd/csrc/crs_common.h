@@ -132,6 +132,47 @@ __device__ __forceinline__ double exp2_64(double t)
     return __builtin_ldexp(p, (int)n);
 }
 
+// ln x in FP64 to about 2 ulp (max relative error 1.81 * 2^-53 measured by
+// exact emulation with a reciprocal seed 2^-23 off).  x = 2^e m with m in
+// [sqrt(1/2), sqrt(2)), f = m - 1 (exact), s = f / (2 + f) from the
+// reciprocal with one correction step that also takes up the rounding of
+// 2 + f, and ln m = 2 s + s^3 P(s^2) with P the degree-7 Chebyshev
+// interpolant of (2 atanh(sqrt z) - 2 sqrt z) / z^(3/2) on
+// [0, (3 - 2 sqrt 2)^2] (coefficients from log64_coeffs.py beside this
+// file); ln x = e ln2 + ln m with ln2 split in two.  About 35 VALU
+// instructions against 90 for the library log.  Positive normal or
+// subnormal x and +inf only (callers clamp with fmax first).
+__device__ __forceinline__ double log64(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);
+    int e = __builtin_amdgcn_frexp_exp(x);
+    const bool lo = m < 0.7071067811865476;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double f = m - 1.0;
+    const double d = 2.0 + f;
+    const double r = rcp64(d);
+    const double q = f * r;
+    const double dl = f - (d - 2.0);
+    double res = __builtin_fma(-q, d, f);
+    res = __builtin_fma(-q, dl, res);
+    const double s = __builtin_fma(res, r, q);
+    const double z = s * s;
+    double p = 0.13086767009948175;
+    p = __builtin_fma(p, z, 0.13268759668560612);
+    p = __builtin_fma(p, z, 0.15386240205813478);
+    p = __builtin_fma(p, z, 0.18181795630884753);
+    p = __builtin_fma(p, z, 0.2222222239180047);
+    p = __builtin_fma(p, z, 0.2857142857080323);
+    p = __builtin_fma(p, z, 0.4000000000000088);
+    p = __builtin_fma(p, z, 0.6666666666666666);
+    const double lm = __builtin_fma(s * z, p, 2.0 * s);
+    const double de = (double)e;
+    const double y = __builtin_fma(de, 0.6931471805599453,
+                                   __builtin_fma(de, 2.3190468138462996e-17, lm));
+    return x == __builtin_inf() ? x : y;
+}
+
 // Per-lane LDS storage.  A copy of an LDS pointer taken through an empty asm
 // is a value the compiler cannot see through: loads made through it can be
 // neither hoisted above the point where the copy is taken nor forwarded from

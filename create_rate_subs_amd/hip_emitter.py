@@ -31,6 +31,7 @@ subtraction and the species sums in FP64.
 import math
 import os
 import re
+import struct
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
 
@@ -620,6 +621,10 @@ def _spec_lines(x, seen, W='w', f='f', r='r', pm='pm'):
     return out
 
 
+# marks the end of the concentration lines in a fused body
+_CONC_DONE = '// concentrations complete'
+
+
 def _conc_lines(ir, K, y='y', C='C'):
     """eval_conc body (rate_subs.py:1292-1391), same operation order."""
     nsp = ir.nsp
@@ -659,13 +664,35 @@ def _indent(lines, n):
 # * the retiring reaction order and the branch-free exp64 add a few %;
 # * base-2 Arrhenius exponents (exp2_64 / exp2_mixed) cut 2 % of the
 #   instructions, within timing noise;
+# * waves spent 27 % of their time in s_waitcnt (SQ_WAIT_ANY), mostly on
+#   each reaction block's per-lane LDS reads; issuing them lds_prefetch
+#   blocks ahead (_prefetch_lane_lds) takes FP64 1.295 -> 1.180 ms and
+#   mixed 0.846 -> 0.711 ms at 53 species with 3 blocks (1: 1.198 / 0.723,
+#   2: 1.187 / 0.719), and 4.13 -> 3.96 / 2.70 -> 2.58 ms at 111 species
+#   with 1 block (2 blocks: 4.09 / 2.66; more live values there spill);
+# * the library log10 / exp10 / exp / pow of the falloff blocks were ~20 %
+#   of the FP64 instructions (log10 alone ~90 each); crs::log64, exp64,
+#   exp2_64 and rcp64 instead: FP64 1.23 -> 1.04 ms at 53 species, 4.1 ->
+#   3.2 ms at 111;
 # * the NASA coefficients as a per-block LDS table selected per lane
-#   (nasa='lds') cut 4.5 % (FP64) / 8 % (mixed) of the instructions but
-#   run 3 % slower (LDS latency at one wave per SIMD): not the default.
+#   (nasa='lds': 7 FMAs and 4 LDS reads per species instead of 14 FMAs and
+#   28 scalar moves) lost 3 % while its reads were waited for at once;
+#   read 4 species ahead it gains 7-9 % (FP64 1.038 -> 0.958 ms, mixed
+#   0.758 -> 0.685 ms at 53 species): the default where the H/G form fits;
+# * the reaction-block literals as a wave-uniform LDS table
+#   (consts_rx='lds' / 'lds_fp64', _block_consts_to_lds) remove 2/3 of the
+#   scalar moves and nearly all scalar spills but cost as many LDS reads
+#   and AGPR copies: 4 % faster FP64 before the changes above, 1-5 %
+#   slower after them; not the default;
+# * the mass-fraction loads are waited for by eval_conc's sums at the top
+#   of the kernel; with conc_late the T-only work (NASA basis, the 53 G_k)
+#   runs first and covers them: FP64 1.033 -> 0.996 ms, mixed 0.737 ->
+#   0.687 ms at 53 species (at 111 species FP64 3.20 -> 3.40 ms: off there).
 # 'auto' picks the H/G form with G in LDS when every species fits the LDS
 # budget (78 rows), otherwise species Kc with 78 concentration rows in LDS.
 DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
-                    fused='fast', kc='auto', exp='crs', order='retire')
+                    fused='fast', kc='auto', exp='crs', order='retire',
+                    nasa='auto')
 
 
 def resolve_opts(nsp, opts=None):
@@ -673,12 +700,18 @@ def resolve_opts(nsp, opts=None):
 
     LDS budget at one wave per SIMD: 160 KB per CU for 4 waves.  Per-lane
     rows take 512 B per wave each; the NASA coefficient table (option
-    nasa='lds') takes 128 B per species per block.
+    nasa='lds') takes 128 B per species per block.  nasa='auto' takes the
+    table where the H/G form fits beside it.
     """
     o = dict(DEFAULT_OPTS, **(opts or {}))
+
+    def rows(table):
+        return min(78, (160 * 1024 - table - 1024) // 2048)
+    if o.get('nasa') == 'auto':
+        fits = nsp <= rows(128 * nsp) and o.get('kc') in ('auto', 'species_hg')
+        o['nasa'] = 'lds' if fits else None
     if 'lds_species' not in o:
-        table = 128 * nsp if o.get('nasa') == 'lds' else 0
-        o['lds_species'] = min(78, (160 * 1024 - table - 1024) // 2048)
+        o['lds_species'] = rows(128 * nsp if o.get('nasa') == 'lds' else 0)
     if o.get('kc') == 'auto':
         if nsp <= o['lds_species']:
             o['kc'] = 'species_hg'
@@ -686,6 +719,10 @@ def resolve_opts(nsp, opts=None):
         else:
             o['kc'] = 'species'
             o.setdefault('c_store', 'lane_lds')
+    o.setdefault('lds_prefetch', 3 if o.get('g_store') == 'lane_lds' else 1)
+    o.setdefault('conc_late', o.get('g_store') == 'lane_lds')
+    if o.get('nasa') == 'lds':
+        o.setdefault('nasa_prefetch', 4)
     return o
 
 
@@ -827,6 +864,27 @@ def _fast_lnkc(x, K):
     return ' + '.join(terms) if terms else '0.0'
 
 
+_LOG10E = '0.4342944819032518'
+_LOG2_10 = '3.321928094887362'
+
+
+def _exp_t(pos, c, K):
+    """exp(-T c) (exp(T c) when not pos): crs::exp64 while |T c| stays
+    below 1e8 for T up to 10^4 K (exp64's reduction is exact there and
+    it over/underflows like exp beyond 709), the library exp otherwise
+    (e.g. T*** = 1e-30 K)."""
+    fn = 'crs::exp64' if abs(c) * 1.0e4 <= 1.0e8 else 'exp'
+    return '{}({}T * {})'.format(fn, '-' if pos else '', K(c))
+
+
+def _exp_it(pos, c, K):
+    """exp(-c / T) as exp(-c * iT); exp64 while |c| / T stays below 1e8
+    for T down to 1 K (and below 7e4 so that the term cannot underflow to
+    zero at T >= 100 K)."""
+    fn = 'crs::exp64' if abs(c) <= 7.0e4 else 'exp'
+    return '{}({}{} * iT)'.format(fn, '-' if pos else '', K(c))
+
+
 def _fast_pm_lines(x, K, kf_x, arr2=None):
     """Pressure modification of reaction x into 'pm' (fast FP64 form).
 
@@ -863,42 +921,48 @@ def _fast_pm_lines(x, K, kf_x, arr2=None):
         out.append('const double Pr = {} * {} / {};'.format(num, coll, den))
     tail = ''
     if x.falloff_low:
-        tail = ' * Pr / (1.0 + Pr)'
+        tail = ' * Pr * crs::rcp64(1.0 + Pr)'
     elif x.falloff_high:
-        tail = ' / (1.0 + Pr)'
+        tail = ' * crs::rcp64(1.0 + Pr)'
     if x.falloff == 'troe':
         t = x.troe
-        e1 = 'exp({}T * {})'.format('-' if t['t3_pos'] else '', K(1.0 / t['t3']))
-        e2 = 'exp({}T * {})'.format('-' if t['t1_pos'] else '', K(1.0 / t['t1']))
+        e1 = _exp_t(t['t3_pos'], 1.0 / t['t3'], K)
+        e2 = _exp_t(t['t1_pos'], 1.0 / t['t1'], K)
         fc = '{} * {} + {} * {}'.format(K(t['one_minus_a']), e1, K(t['a']), e2)
         if t['t2'] is not None:
-            fc += ' + exp({}{} * iT)'.format('-' if t['t2_pos'] else '',
-                                             K(t['t2']))
-        out.append('const double logFcent = log10(fmax({}, 1.0e-300));'.format(fc))
-        out.append('const double lPr = log10(fmax(Pr, 1.0e-300));')
+            fc += ' + ' + _exp_it(t['t2_pos'], t['t2'], K)
+        out.append('const double logFcent = crs::log64(fmax({}, 1.0e-300)) * '
+                   '{};'.format(fc, _LOG10E))
+        out.append('const double lPr = crs::log64(fmax(Pr, 1.0e-300)) * '
+                   '{};'.format(_LOG10E))
         out.append('const double A = lPr - 0.67 * logFcent - 0.4;')
         out.append('const double B = 0.806 - 1.1762 * logFcent - 0.14 * lPr;')
-        out.append('const double pm = exp10(logFcent / (1.0 + A * A / (B * B))){};'
-                   .format(tail))
+        # 10^(logFcent / (1 + A^2 / B^2)) = 2^(log2(10) logFcent B^2 / (B^2 + A^2))
+        out.append('const double pm = crs::exp2_64(logFcent * (B * B) * '
+                   'crs::rcp64(B * B + A * A) * {}){};'.format(_LOG2_10, tail))
     elif x.falloff == 'sri':
         sr = x.sri
-        out.append('const double lPr = log10(fmax(Pr, 1.0e-300));')
-        out.append('const double X = 1.0 / (1.0 + lPr * lPr);')
-        e1 = 'exp({}{} * iT)'.format('-' if sr['b_pos'] else '', K(sr['b']))
-        e2 = 'exp({}T * {})'.format('-' if sr['c_pos'] else '', K(1.0 / sr['c']))
+        out.append('const double lPr = crs::log64(fmax(Pr, 1.0e-300)) * '
+                   '{};'.format(_LOG10E))
+        out.append('const double X = crs::rcp64(1.0 + lPr * lPr);')
+        e1 = _exp_it(sr['b_pos'], sr['b'], K)
+        e2 = _exp_t(sr['c_pos'], 1.0 / sr['c'], K)
         base = '{} * {} + {}'.format(K(sr['a']), e1, e2)
-        if sr['a'] > 0.0:
+        if sr['a'] > 0.0 and e1.startswith('crs::') and e2.startswith('crs::'):
+            # a e^(-b/T) > 0 at every T >= 100 K here, so ln(base) is finite
+            f = 'crs::exp64(X * crs::log64({}))'.format(base)
+        elif sr['a'] > 0.0:
             f = 'exp(X * log({}))'.format(base)
         else:
             f = 'pow({}, X)'.format(base)
         if sr['d'] is not None:
-            f += ' * {} * exp({} * logT)'.format(K(sr['d']), K(sr['e']))
+            f += ' * {} * crs::exp64({} * logT)'.format(K(sr['d']), K(sr['e']))
         out.append('const double pm = {}{};'.format(f, tail))
     else:
         if x.falloff_low:
-            out.append('const double pm = Pr / (1.0 + Pr);')
+            out.append('const double pm = Pr * crs::rcp64(1.0 + Pr);')
         elif x.falloff_high:
-            out.append('const double pm = 1.0 / (1.0 + Pr);')
+            out.append('const double pm = crs::rcp64(1.0 + Pr);')
         else:
             out.append('const double pm = 1.0;')
     return out
@@ -1000,14 +1064,31 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
         out.append('volatile double (*sG)[crs::kBlock] = sG_;')
         out.append('const int lx = threadIdx.x;')
     slots = {k: s for s, k in enumerate(_gibbs_species(ir))}
-    for k in sorted(need_g | need_gi):
+    order = sorted(need_g | need_gi)
+
+    def nasa_reads(k):
+        # this lane's range of species k's row of the coefficient table
+        g = tmids.index(ir.nasa[k][0]) if ir.nasa[k][0] in tmids else None
+        return (['CRS_LDS double *const np{k} = crs::opaque_lds((CRS_LDS '
+                 'double *)(crs_nasa_lds + {o} + nsel{g}));'.format(
+                     k=k, o=16 * k, g=g)] +
+                ['const double nc{k}_{i} = np{k}[{i}];'.format(k=k, i=i)
+                 for i in range(7)])
+    for pos, k in enumerate(order):
         sp = ir.nasa[k]
         gi = tmids.index(sp[0]) if sp[0] in tmids else None
         if nasa_lds:
-            # this lane's range of the coefficient table in LDS
-            out.append('const double smh{k} = crs::nasa_smh(crs::opaque_lds('
-                       '(CRS_LDS double *)(crs_nasa_lds + {o} + nsel{g})), '
-                       'logT, T, T2, T3, T4, iT);'.format(k=k, o=16 * k, g=gi))
+            # rows are read nasa_lds blocks ahead of their use (an int >= 1)
+            # so that the LDS latency overlaps the previous exponentials
+            ahead = 0 if nasa_lds is True else int(nasa_lds)
+            if pos == 0:
+                for kk in order[:ahead]:
+                    out += nasa_reads(kk)
+            if pos + ahead < len(order):
+                out += nasa_reads(order[pos + ahead])
+            out.append('const double smh{k} = nc{k}_0 + nc{k}_1 * logT + '
+                       'nc{k}_2 * T + nc{k}_3 * T2 + nc{k}_4 * T3 + '
+                       'nc{k}_5 * T4 + nc{k}_6 * iT;'.format(k=k))
         else:
             terms = []
             for half, a in (('l', sp[1]), ('h', sp[2])):
@@ -1118,9 +1199,9 @@ def _mixed_pm_lines(x, K, kf_x):
         lpr = '(float)log10(fmax(Pr, 1.0e-300))'
     ratio = ''
     if x.falloff_low:
-        ratio = ' * (Pr / (1.0 + Pr))'
+        ratio = ' * (Pr * crs::rcp64(1.0 + Pr))'
     elif x.falloff_high:
-        ratio = ' / (1.0 + Pr)'
+        ratio = ' * crs::rcp64(1.0 + Pr)'
     if x.falloff in ('troe', 'sri'):
         out.append('const float lPr = {};'.format(lpr))
     if x.falloff == 'troe':
@@ -1157,9 +1238,9 @@ def _mixed_pm_lines(x, K, kf_x):
         out.append('const double pm = (double)F{};'.format(ratio))
     else:
         if x.falloff_low:
-            out.append('const double pm = Pr / (1.0 + Pr);')
+            out.append('const double pm = Pr * crs::rcp64(1.0 + Pr);')
         elif x.falloff_high:
-            out.append('const double pm = 1.0 / (1.0 + Pr);')
+            out.append('const double pm = crs::rcp64(1.0 + Pr);')
         else:
             out.append('const double pm = 1.0;')
     return out
@@ -1181,7 +1262,7 @@ def _concentrations_to_lds(lines, nsp):
             out.append('__shared__ double sC[{}][crs::kBlock];'.format(nsp))
             out.append('const int lxc = threadIdx.x;')
             continue
-        if l.startswith('const double logT = log(T);'):
+        if l == _CONC_DONE:
             # concentrations complete: publish them, then read back
             out.append('__syncthreads();')
         out.append(_C_REF.sub(r'sC[\1][lxc]', l))
@@ -1209,7 +1290,7 @@ def _concentrations_to_lane_lds(lines, nsp, nl):
     out = []
     published = False
     for l in lines:
-        if not published and l.startswith('const double logT = log(T);'):
+        if not published and l == _CONC_DONE:
             out.append('__shared__ double sCl[crs::kBlock / 64][{}][64];'
                        .format(max(len(slot), 1)))
             out.append('CRS_LDS double *const cb = (CRS_LDS double *)'
@@ -1228,6 +1309,132 @@ def _concentrations_to_lane_lds(lines, nsp, nl):
         out.append(l)
         if l.startswith('{  // rxn'):
             out.append('    CRS_LDS double *const cl = crs::opaque_lds(cb);')
+    return out
+
+
+def _prefetch_lane_lds(lines, sources, dist):
+    """Issue each reaction block's LDS reads ``dist`` blocks early.
+
+    Input: blocks ``{  // rxn ..`` .. ``}`` whose first lines take
+    ``ref = crs::opaque_lds(base)`` for every (base, ref, per_lane) of
+    ``sources`` and which read ``ref[k * 64]`` (per-lane rows) or
+    ``ref[k]`` (the block-wide constant table).  The reads of block j move
+    to the outer scope in front of block j - dist, each group behind its
+    own opaque copy of the base pointer, and the block uses the loaded
+    values.  With one wave per SIMD nothing else covers an LDS read's
+    latency; issued a block ahead, the reads complete while the previous
+    reaction's exponential runs.  Values stay live for ``dist`` blocks (a
+    few registers).
+    """
+    srcs = []
+    for src in sources:
+        base, ref, per_lane = src[:3]
+        d = src[3] if len(src) > 3 else dist
+        take = '    CRS_LDS double *const {} = crs::opaque_lds({});'.format(
+            ref, base)
+        pat = re.compile(r'\b{}\[(\d+){}\]'.format(
+            re.escape(ref), r' \* 64' if per_lane else ''))
+        srcs.append((base, ref, per_lane, take, pat, d))
+    takes = {t for _b, _r, _l, t, _p, _d in srcs}
+    starts = [p for p, l in enumerate(lines) if l.startswith('{  // rxn')]
+    blocks = []   # (start, end, refs per source)
+    for s in starts:
+        e = s + 1
+        while lines[e] != '}':
+            e += 1
+        refs = [sorted({int(mo.group(1)) for l in lines[s:e]
+                        for mo in pat.finditer(l)})
+                for _b, _r, _l, _t, pat, _d in srcs]
+        blocks.append((s, e, refs))
+
+    def group(j, d):
+        """Reads of block j for the sources prefetched d blocks ahead."""
+        out = []
+        for (base, ref, per_lane, _t, _p, sd), refs in zip(srcs,
+                                                           blocks[j][2]):
+            if not refs or sd != d:
+                continue
+            p = '{}p{}'.format(ref, j)
+            out.append('CRS_LDS double *const {} = crs::opaque_lds({});'
+                       .format(p, base))
+            out += ['const double {r}{j}_{k} = {p}[{k}{m}];'.format(
+                r=ref, j=j, k=k, p=p, m=' * 64' if per_lane else '')
+                    for k in refs]
+        return out
+
+    dists = sorted({sd for *_x, sd in srcs}, reverse=True)
+    out = []
+    prev = 0
+    for j, (s, e, _refs) in enumerate(blocks):
+        out += lines[prev:s]
+        for d in dists:
+            if j == 0:
+                for jj in range(min(d, len(blocks))):
+                    out += group(jj, d)
+            if j + d < len(blocks):
+                out += group(j + d, d)
+        for l in lines[s:e + 1]:
+            if l in takes:
+                continue
+            for _b, ref, _l, _t, pat, _d in srcs:
+                l = pat.sub(lambda mo, r=ref: '{}{}_{}'.format(
+                    r, j, mo.group(1)), l)
+            out.append(l)
+        prev = e + 1
+    out += lines[prev:]
+    return out
+
+
+# FP64 operands that need no scalar moves: the hardware's inline
+# constants, and values whose low 32 bits are zero (one 32-bit literal).
+_INLINE_F64 = {0.0, 0.5, 1.0, 2.0, 4.0}
+_FLT_LIT = re.compile(r'(?<![\w.])(\d+\.\d+(?:e[+-]?\d+)?|\d+e[+-]?\d+)'
+                      r'(?![\w.])')
+
+
+def _cheap_f64(v):
+    if v in _INLINE_F64:
+        return True
+    return struct.unpack('<Q', struct.pack('<d', v))[0] & 0xffffffff == 0
+
+
+def _block_consts_to_lds(lines, table, base='ktb', ref='kt'):
+    """Move the FP64 literals of each reaction block into an LDS table.
+
+    A literal costs two scalar moves; at one wave per SIMD those take issue
+    slots from the wave's own vector work, and thousands of them also
+    exhaust the scalar registers (spilled to vector lanes).  Every literal
+    inside a block that is not an inline constant becomes a slot of
+    ``table`` (appended to; slots of one block are contiguous, a value
+    repeated within a block shares its slot), read as a wave-uniform LDS
+    broadcast through ``ref = crs::opaque_lds(base)`` at the top of the
+    block.  Values are unchanged (the table holds the same doubles).
+    """
+    out = []
+    inside = False
+    slots = {}
+    for l in lines:
+        if l.startswith('{  // rxn'):
+            inside = True
+            slots = {}
+            out.append(l)
+            out.append('    CRS_LDS double *const {} = crs::opaque_lds({});'
+                       .format(ref, base))
+            continue
+        if inside and l == '}':
+            inside = False
+        if inside and not l.lstrip().startswith('CRS_LDS'):
+            def sub(mo):
+                v = float(mo.group(1))
+                if _cheap_f64(v):
+                    return mo.group(0)
+                key = v.hex()
+                if key not in slots:
+                    slots[key] = len(table)
+                    table.append(v)
+                return '{}[{}]'.format(ref, slots[key])
+            l = _FLT_LIT.sub(sub, l)
+        out.append(l)
     return out
 
 
@@ -1362,7 +1569,8 @@ def _dydt_finish_lines(ir, K, tmids, k, wk):
     return out
 
 
-def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
+def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
+                      ktab_name=None):
     """Per-state body of the fused FP64 kernel, arranged for the GPU.
 
     Same physics and reaction order as the reference, with the arithmetic
@@ -1391,15 +1599,20 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
                  'double y[{}];'.format(max(nsp - 1, 1))]
         lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k)
                   for k in range(nsp - 1)]
-    lines += _conc_lines(ir, K)
-    lines += ['const double logT = log(T);',
+    # conc_late: the T-only work (basis, G_k) runs while the mass fractions
+    # are still in flight; eval_conc's sums wait for them afterwards
+    conc = _conc_lines(ir, K) + [_CONC_DONE]
+    late = opts.get('conc_late', False)
+    if not late:
+        lines += conc
+    lines += ['const double logT = crs::log64(T);',
               'const double iT = 1.0 / T;',
               'const double T2 = T * T;', 'const double T3 = T2 * T;',
               'const double T4 = T3 * T;', 'const double T5 = T4 * T;',
               'const double m = P / ({} * T);'.format(ru),
               '(void)m; (void)T2; (void)T3; (void)T4; (void)T5;']
     if any(x.plog or x.cheb for x in ir.rxns):
-        lines += ['const double logP = log(P);',
+        lines += ['const double logP = crs::log64(P);',
                   'const double log10P = logP * 0.43429448190325182;',
                   '(void)log10P;']
     lines += ['const double log2T = logT * 1.4426950408889634;',
@@ -1431,13 +1644,18 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
                 g=gi, b=bi, v=bname))
             lines.append('const double ph{g}_{b} = lo{g} ? 0.0 : {v};'.format(
                 g=gi, b=bi, v=bname))
+    dydt_lines = []
     if dydt:
-        lines += _dydt_cp_lines(ir, K, tmids)
-        lines += ['const double irho = 1.0 / rho;', 'double esum = 0.0;']
+        dydt_lines = _dydt_cp_lines(ir, K, tmids) + [
+            'const double irho = 1.0 / rho;', 'double esum = 0.0;']
+        if not late:
+            lines += dydt_lines
     hg = opts.get('kc') == 'species_hg'
     species_kc = opts.get('kc') in ('species', 'species_hg')
     nasa_lds = opts.get('nasa') == 'lds'
     if nasa_lds:
+        nasa_lds = max(1, int(opts.get('nasa_prefetch', 2))) if opts.get(
+            'nasa_prefetch', 2) else True
         lines += ['const int nsel{g} = lo{g} ? 0 : 8;'.format(g=gi)
                   for gi in range(len(tmids))]
     if species_kc:
@@ -1454,16 +1672,18 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
     n_gl = min(nsp, opts.get('lds_species', 78)) if g_lane else 0
     gref = (lambda k: 'gl[{} * 64]'.format(k) if k < n_gl else
             'G{}'.format(k))
+    if hg and g_lane:
+        lines.append('__shared__ double sGl[crs::kBlock / 64][{}][64];'
+                     .format(n_gl))
+        lines.append('CRS_LDS double *const gb = (CRS_LDS double *)'
+                     '&sGl[threadIdx.x >> 6][0][threadIdx.x & 63];')
+        lines += ['gb[{k} * 64] = G{k};'.format(k=k) for k in range(n_gl)]
+    if late:
+        lines += conc + dydt_lines
     if hg:
         # H_k = C_k / G_k; from here on C_k is G_k H_k (no third array)
         lines += ['const double H{k} = C[{k}] * crs::rcp64(G{k});'.format(k=k)
                   for k in range(nsp)]
-        if g_lane:
-            lines.append('__shared__ double sGl[crs::kBlock / 64][{}][64];'
-                         .format(n_gl))
-            lines.append('CRS_LDS double *const gb = (CRS_LDS double *)'
-                         '&sGl[threadIdx.x >> 6][0][threadIdx.x & 63];')
-            lines += ['gb[{k} * 64] = G{k};'.format(k=k) for k in range(n_gl)]
     rx_start = len(lines)
     rxns = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
     last = _last_touch(ir, rxns)
@@ -1628,11 +1848,24 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False):
                                'crs::opaque_lds(gb);')
             rx = out
         lines = lines[:rx_start] + rx
+    sources = [('gb', 'gl', True)] if hg and g_lane else []
     if opts.get('c_store') == 'lds':
         lines = _concentrations_to_lds(lines, ir.nsp)
     elif opts.get('c_store') == 'lane_lds':
         lines = _concentrations_to_lane_lds(
             lines, ir.nsp, min(ir.nsp, opts.get('lds_species', 78)))
+        sources.append(('cb', 'cl', True))
+    if ktab is not None:
+        first = next(p for p, l in enumerate(lines)
+                     if l.startswith('{  // rxn'))
+        lines = (lines[:first] +
+                 ['CRS_LDS double *const ktb = (CRS_LDS double *){};'.format(
+                     ktab_name)] +
+                 _block_consts_to_lds(lines[first:], ktab))
+        sources.append(('ktb', 'kt', False,
+                        opts.get('consts_prefetch', opts.get('lds_prefetch'))))
+    if opts.get('lds_prefetch') and sources:
+        lines = _prefetch_lane_lds(lines, sources, opts['lds_prefetch'])
     return lines
 
 
@@ -1753,20 +1986,48 @@ def emit_hip_source(ir, opts=None):
             len(vals)) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
                                                else ' ')
                                  for q, v in enumerate(vals)) + '};\n')
-        body.append('__shared__ double crs_nasa_lds[{}];\n\n'.format(len(vals)))
+        body.append('__shared__ __align__(16) double crs_nasa_lds[{}];\n\n'.format(
+            len(vals)))
         nasa_fill = ('    for (int q = threadIdx.x; q < {n}; q += blockDim.x)\n'
                      '        crs_nasa_lds[q] = crs_nasa_tab[q];\n'
                      '    __syncthreads();\n'.format(n=len(vals)))
 
+    def fused_state(name, o, **kw):
+        """Lines of one fused per-state body and the fill code of its
+        kernel; with consts_rx='lds' (every fused kernel) or 'lds_fp64'
+        (the FP64 ones) the body's reaction-block literals live in an LDS
+        table (declared here, filled by the kernel)."""
+        mode = opts.get('consts_rx')
+        if mode not in ('lds', 'lds_fp64') or (mode == 'lds_fp64' and
+                                                name == 'mixed'):
+            return _fused_fast_lines(ir, K, o, **kw), nasa_fill
+        tab = []
+        lines = _fused_fast_lines(ir, K, o, ktab=tab,
+                                  ktab_name='crs_kt_' + name, **kw)
+        n = max(len(tab), 1)
+        body.append('__device__ const double crs_kt_{}_src[{}] = {{\n'.format(
+            name, n) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
+                                             else ' ')
+                               for q, v in enumerate(tab or [0.0])) + '};\n')
+        body.append('__shared__ __align__(16) double crs_kt_{}[{}];\n\n'.format(
+            name, n))
+        fill = ('    for (int q = threadIdx.x; q < {n}; q += blockDim.x)\n'
+                '        crs_kt_{name}[q] = crs_kt_{name}_src[q];\n'
+                '    __syncthreads();\n'.format(n=n, name=name))
+        return lines, nasa_fill + fill
+
     # ---- fused FP64 per-state body ------------------------------------
+    if opts['fused'] == 'fast':
+        f64_lines, f64_fill = fused_state('f64', opts)
+    else:
+        f64_lines, f64_fill = _fused_f64_lines(ir, K, opts), nasa_fill
     body.append(
         '__device__ __forceinline__ void wdot_f64_state(\n'
         '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
         '    int64_t ld, int64_t i)\n{\n' +
-        _indent((_fused_fast_lines if opts['fused'] == 'fast' else
-                 _fused_f64_lines)(ir, K, opts), 4) + '}\n\n')
+        _indent(f64_lines, 4) + '}\n\n')
     # one kernel serves the full batch (idx == nullptr) and subsets
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_wdot_f64(\n'
@@ -1774,7 +2035,7 @@ def emit_hip_source(ir, opts=None):
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
         '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
-        nasa_fill +
+        f64_fill +
         '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
         '    if (t >= m) return;\n'
         '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
@@ -1782,18 +2043,19 @@ def emit_hip_source(ir, opts=None):
         '    wdot_f64_state(K, T_, P_, Y, W, ld, i);\n}\n\n')
 
     # ---- constant-pressure dydt (FP64), the caller of the path -------
+    dydt_lines, dydt_fill = fused_state(
+        'dydt', dict(opts, contraction='sparse'), dydt=True)
     body.append(
         '__device__ __forceinline__ void dydt_f64_state(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
         '    int64_t ld, int64_t i)\n{\n' +
-        _indent(_fused_fast_lines(ir, K, dict(opts, contraction='sparse'),
-                                  dydt=True), 4) + '}\n\n')
+        _indent(dydt_lines, 4) + '}\n\n')
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_dydt_f64(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
-        '    int64_t ld)\n{\n' + nasa_fill +
+        '    int64_t ld)\n{\n' + dydt_fill +
         '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
         '    if (i >= n) return;\n'
         '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
@@ -1837,6 +2099,8 @@ def emit_hip_source(ir, opts=None):
         '        t_ign[i] = t_new;\n}\n\n')
 
     # ---- fused mixed-precision per-state body -------------------------
+    if opts.get('mixed', True):
+        mixed_lines, mixed_fill = fused_state('mixed', opts, mixed=True)
     if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
         rx = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
         nst, nch, vals = _mfma_nu_table(ir, rx)
@@ -1851,15 +2115,14 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
             '    int64_t ld, int64_t i, bool live)\n{\n' +
-            _indent(_fused_fast_lines(ir, K, opts, mixed=True), 4) +
-            '}\n\n')
+            _indent(mixed_lines, 4) + '}\n\n')
         body.append(
             '__global__ __launch_bounds__(crs::kBlock) void k_wdot_mixed(\n'
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
             '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
-            + nasa_fill +
+            + mixed_fill +
             '    // whole waves beyond m leave; within a wave every lane runs (the\n'
             '    // contraction exchanges data across lanes), dead lanes on state 0\n'
             '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
@@ -1875,15 +2138,14 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
             '    int64_t ld, int64_t i)\n{\n' +
-            _indent(_fused_fast_lines(ir, K, opts, mixed=True), 4) +
-            '}\n\n')
+            _indent(mixed_lines, 4) + '}\n\n')
         body.append(
             '__global__ __launch_bounds__(crs::kBlock) void k_wdot_mixed(\n'
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
             '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
-            nasa_fill +
+            mixed_fill +
             '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
             '    if (t >= m) return;\n'
             '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'

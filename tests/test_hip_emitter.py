@@ -1,0 +1,112 @@
+"""Text-level checks of the HIP emitter's fused kernels (CPU, no hipcc).
+
+The GPU parity tests prove the numbers; these pin the layout decisions the
+numbers depend on, so that a change which silently drops one shows up here
+before a GPU run: per-lane LDS rows are used at both sizes (G rows when
+every species fits, concentration rows otherwise), each reaction block's
+LDS reads are issued ahead of it, the fused FP64 bodies call no library
+transcendental, and the LDS constant-table pass keeps every value.
+"""
+
+import re
+
+import pytest
+
+from create_rate_subs_amd import hip_emitter, workloads
+from create_rate_subs_amd.create_rate_subs import load_mechanism
+
+_BODIES = ('wdot_f64_state', 'wdot_mixed_state', 'dydt_f64_state')
+
+
+def _ir(name):
+    mpath, tpath = workloads.mechanism_files(name)
+    _e, specs, reacs, _fwd = load_mechanism(mpath, tpath)
+    return hip_emitter.build_ir(specs, reacs, name=name)
+
+
+def _bodies(name, opts=None):
+    text = hip_emitter.emit_hip_source(_ir(name), opts)
+    out = {}
+    for fn in _BODIES:
+        i = text.index('void ' + fn)
+        out[fn] = text[i:text.index('\n}\n', i)]
+    return text, out
+
+
+@pytest.mark.parametrize('name,rows,ref', [('mech53', 'sGl', 'gl'),
+                                           ('mech111', 'sCl', 'cl')])
+def test_lane_lds_rows_and_prefetch(name, rows, ref):
+    _text, bodies = _bodies(name)
+    for fn, b in bodies.items():
+        assert '__shared__ double {}['.format(rows) in b, fn
+        # every block's reads come from a prefetch group outside the block
+        blocks = re.findall(r'\{  // rxn \d+\n(.*?)\n    \}', b, re.S)
+        assert blocks, fn
+        assert not any(re.search(r'\b{}\['.format(ref), blk) for blk in blocks)
+        assert re.search(r'const double {}\d+_\d+ = {}p\d+\['.format(ref, ref), b)
+        # the NASA table rides along where the H/G rows leave room for it
+        assert ('crs_nasa_lds' in b) == (rows == 'sGl'), fn
+
+
+@pytest.mark.parametrize('name', ['mech53', 'mech111', 'mech21pc'])
+def test_fused_fp64_bodies_call_no_library_transcendental(name):
+    _text, bodies = _bodies(name)
+    for fn in ('wdot_f64_state', 'dydt_f64_state'):
+        calls = re.findall(r'(?<![\w:])(exp|exp10|log|log10|pow)\(',
+                           bodies[fn])
+        assert calls == [], (fn, sorted(set(calls)))
+
+
+def test_prefetch_moves_every_read_ahead():
+    lines = ['double w[3];']
+    for j in range(5):
+        lines += ['{  // rxn %d' % j,
+                  '    CRS_LDS double *const gl = crs::opaque_lds(gb);',
+                  '    const double q = gl[{} * 64] * gl[{} * 64];'.format(
+                      j, (j + 2) % 5),
+                  '    w[0] = q;', '}', 'W[i] = w[0];']
+    for dist in (0, 1, 3, 7):
+        out = hip_emitter._prefetch_lane_lds(lines, [('gb', 'gl', True)],
+                                             dist)
+        defined = {}
+        for p, l in enumerate(out):
+            mo = re.match(r'const double (gl\d+_\d+) = glp(\d+)\[(\d+) \* 64\];',
+                          l)
+            if mo:
+                defined[mo.group(1)] = (p, int(mo.group(2)), int(mo.group(3)))
+        starts = [p for p, l in enumerate(out) if l.startswith('{  // rxn')]
+        for j, s in enumerate(starts):
+            used = set(re.findall(r'gl\d+_\d+', out[s + 1]))
+            assert used == {'gl{}_{}'.format(j, j),
+                            'gl{}_{}'.format(j, (j + 2) % 5)}
+            for v in used:
+                p, blk, _k = defined[v]
+                assert blk == j and p < s
+                # issued before the block dist places earlier (or the top)
+                if j - dist >= 0:
+                    assert p < starts[j - dist]
+        assert sum(l.startswith('W[i]') for l in out) == 5
+
+
+def test_constant_table_keeps_every_value():
+    lines = ['{  // rxn 0',
+             '    const double x = 28.279682783733417 + -0.196 * log2T - '
+             '19939.52130381234 * iT;',
+             '    const double y = 2.0 * x + 0.5 - 1.0e-300 + 0.196;',
+             '}',
+             'const double z = 3.25;']
+    table = []
+    out = hip_emitter._block_consts_to_lds(lines, table)
+    assert out[1] == '    CRS_LDS double *const kt = crs::opaque_lds(ktb);'
+    body = '\n'.join(out)
+    # inline constants stay; 0.196 shares its slot; outside a block unchanged
+    assert '2.0 * x + 0.5' in body and out[-1] == 'const double z = 3.25;'
+    assert table == [28.279682783733417, 0.196, 19939.52130381234, 1e-300]
+
+    def back(mo):
+        return repr(table[int(mo.group(1))])
+    restored = [re.sub(r'kt\[(\d+)\]', back, l) for l in out[2:4]]
+    num = r'\d+\.\d+(?:e[+-]?\d+)?|\d+e[+-]?\d+'
+    nums = [float(v) for l in restored for v in re.findall(num, l)]
+    orig = [float(v) for l in lines[1:3] for v in re.findall(num, l)]
+    assert nums == orig

@@ -1,0 +1,62 @@
+"""crs::log64 (the branch-free FP64 logarithm of the fused kernels).
+
+The coefficients in crs_common.h must be the ones log64_coeffs.py derives
+(Chebyshev interpolant of (2 atanh(sqrt z) - 2 sqrt z) / z^(3/2)), and the
+device sequence, emulated with one rounding per operation and a reciprocal
+seed 2^-23 off in either direction, must stay within 2 units of 2^-53
+relative of ln x (mpmath, 60 digits) from 1e-300 to 1e300 and next to 1.
+A dropped series term or a wrong ln2 split fails the bound.
+"""
+
+import importlib.util
+import os
+import re
+
+import mpmath as mp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, 'create_rate_subs_amd', 'csrc')
+
+
+def _coeffs_module():
+    spec = importlib.util.spec_from_file_location(
+        'log64_coeffs', os.path.join(CSRC, 'log64_coeffs.py'))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _device_constants():
+    text = open(os.path.join(CSRC, 'crs_common.h')).read()
+    body = text[text.index('double log64(double x)'):]
+    body = body[:body.index('\n}\n')]
+    return [float(v) for v in re.findall(r'(-?\d+\.\d+(?:e-?\d+)?)', body)]
+
+
+def test_header_holds_derived_coefficients():
+    mod = _coeffs_module()
+    consts = _device_constants()
+    hi, lo = mod.ln2_split()
+    # sqrt(1/2) threshold, the 1.0 / 2.0 of f and 2 + f, P, the ln2 split
+    assert consts[0] == mod.SQRT_HALF
+    cs = mod.coefficients()
+    k = consts.index(cs[0])
+    assert consts[k:k + len(cs)] == cs
+    assert consts[-2:] == [hi, lo]
+
+
+def test_emulated_sequence_error_bound():
+    mod = _coeffs_module()
+    assert mod.max_error(samples=1200, seed=11) <= 2.0
+
+
+def test_bound_catches_a_dropped_term():
+    mod = _coeffs_module()
+    cs = mod.coefficients()
+    hi, lo = mod.ln2_split()
+    x = 1.37
+    good = mod.emulate(x, cs, hi, lo)
+    bad = mod.emulate(x, cs[:-1] + [0.6], hi, lo)
+    ex = mp.log(mp.mpf(x))
+    assert abs(good - ex) / ex < 2 * 2.0 ** -53
+    assert abs(bad - ex) / ex > 1e-6
