@@ -141,8 +141,9 @@ __device__ __forceinline__ double exp2_64(double t)
 // [0, (3 - 2 sqrt 2)^2] (coefficients from log64_coeffs.py beside this
 // file); ln x = e ln2 + ln m with ln2 split in two.  About 35 VALU
 // instructions against 90 for the library log.  Positive normal or
-// subnormal x and +inf only (callers clamp with fmax first).
-__device__ __forceinline__ double log64(double x)
+// subnormal finite x (callers clamp with fmax first); +inf gives NaN, which
+// log64 below turns into +inf.
+__device__ __forceinline__ double log64_finite(double x)
 {
     double m = __builtin_amdgcn_frexp_mant(x);
     int e = __builtin_amdgcn_frexp_exp(x);
@@ -168,8 +169,14 @@ __device__ __forceinline__ double log64(double x)
     p = __builtin_fma(p, z, 0.6666666666666666);
     const double lm = __builtin_fma(s * z, p, 2.0 * s);
     const double de = (double)e;
-    const double y = __builtin_fma(de, 0.6931471805599453,
-                                   __builtin_fma(de, 2.3190468138462996e-17, lm));
+    return __builtin_fma(de, 0.6931471805599453,
+                         __builtin_fma(de, 2.3190468138462996e-17, lm));
+}
+
+// log64_finite with ln(+inf) = +inf, for arguments that may overflow.
+__device__ __forceinline__ double log64(double x)
+{
+    const double y = log64_finite(x);
     return x == __builtin_inf() ? x : y;
 }
 

@@ -666,28 +666,27 @@ def _indent(lines, n):
 #   instructions, within timing noise;
 # * waves spent 27 % of their time in s_waitcnt (SQ_WAIT_ANY), mostly on
 #   each reaction block's per-lane LDS reads; issuing them lds_prefetch
-#   blocks ahead (_prefetch_lane_lds) takes FP64 1.295 -> 1.180 ms and
-#   mixed 0.846 -> 0.711 ms at 53 species with 3 blocks (1: 1.198 / 0.723,
-#   2: 1.187 / 0.719), and 4.13 -> 3.96 / 2.70 -> 2.58 ms at 111 species
-#   with 1 block (2 blocks: 4.09 / 2.66; more live values there spill);
+#   blocks ahead (_prefetch_lane_lds) fixes most of it;
 # * the library log10 / exp10 / exp / pow of the falloff blocks were ~20 %
 #   of the FP64 instructions (log10 alone ~90 each); crs::log64, exp64,
-#   exp2_64 and rcp64 instead: FP64 1.23 -> 1.04 ms at 53 species, 4.1 ->
-#   3.2 ms at 111;
+#   exp2_64 and rcp64 instead: FP64 1.23 -> 1.05 ms at 53 species, 4.05 ->
+#   3.20 ms at 111 (the mixed kernel already used FP32 hardware there);
 # * the NASA coefficients as a per-block LDS table selected per lane
 #   (nasa='lds': 7 FMAs and 4 LDS reads per species instead of 14 FMAs and
-#   28 scalar moves) lost 3 % while its reads were waited for at once;
-#   read 4 species ahead it gains 7-9 % (FP64 1.038 -> 0.958 ms, mixed
-#   0.758 -> 0.685 ms at 53 species): the default where the H/G form fits;
+#   28 scalar moves), read 4 species ahead: the default where the H/G form
+#   fits (at 111 species it costs concentration rows: 3.22 -> 3.45 ms);
+# * the T-only work (NASA basis, the G_k) ahead of eval_conc's sums, which
+#   wait for the mass-fraction loads (conc_late): on where the H/G form
+#   fits (at 111 species FP64 3.20 -> 3.40 ms: off there).
+#   Interleaved A/B at 53 species (5 rounds, median; FP64 / mixed ms):
+#   defaults 0.941 / 0.667; without the LDS prefetch 1.024 / 0.774, with 1
+#   block instead of 3 0.944 / 0.681 (4-8 blocks within 1 %); without the
+#   NASA table 0.976 / 0.704; without conc_late 0.944 / 0.680;
 # * the reaction-block literals as a wave-uniform LDS table
 #   (consts_rx='lds' / 'lds_fp64', _block_consts_to_lds) remove 2/3 of the
 #   scalar moves and nearly all scalar spills but cost as many LDS reads
 #   and AGPR copies: 4 % faster FP64 before the changes above, 1-5 %
-#   slower after them; not the default;
-# * the mass-fraction loads are waited for by eval_conc's sums at the top
-#   of the kernel; with conc_late the T-only work (NASA basis, the 53 G_k)
-#   runs first and covers them: FP64 1.033 -> 0.996 ms, mixed 0.737 ->
-#   0.687 ms at 53 species (at 111 species FP64 3.20 -> 3.40 ms: off there).
+#   slower after them; not the default.
 # 'auto' picks the H/G form with G in LDS when every species fits the LDS
 # budget (78 rows), otherwise species Kc with 78 concentration rows in LDS.
 DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
@@ -931,10 +930,12 @@ def _fast_pm_lines(x, K, kf_x, arr2=None):
         fc = '{} * {} + {} * {}'.format(K(t['one_minus_a']), e1, K(t['a']), e2)
         if t['t2'] is not None:
             fc += ' + ' + _exp_it(t['t2_pos'], t['t2'], K)
-        out.append('const double logFcent = crs::log64(fmax({}, 1.0e-300)) * '
-                   '{};'.format(fc, _LOG10E))
-        out.append('const double lPr = crs::log64(fmax(Pr, 1.0e-300)) * '
-                   '{};'.format(_LOG10E))
+        # Fcent <= 3 is finite; an infinite Pr makes pm NaN in the reference
+        # too (inf / inf in A^2 / B^2), so neither log needs the inf guard
+        out.append('const double logFcent = crs::log64_finite(fmax({}, '
+                   '1.0e-300)) * {};'.format(fc, _LOG10E))
+        out.append('const double lPr = crs::log64_finite(fmax(Pr, 1.0e-300)) '
+                   '* {};'.format(_LOG10E))
         out.append('const double A = lPr - 0.67 * logFcent - 0.4;')
         out.append('const double B = 0.806 - 1.1762 * logFcent - 0.14 * lPr;')
         # 10^(logFcent / (1 + A^2 / B^2)) = 2^(log2(10) logFcent B^2 / (B^2 + A^2))
@@ -1605,14 +1606,14 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     late = opts.get('conc_late', False)
     if not late:
         lines += conc
-    lines += ['const double logT = crs::log64(T);',
+    lines += ['const double logT = crs::log64_finite(T);',
               'const double iT = 1.0 / T;',
               'const double T2 = T * T;', 'const double T3 = T2 * T;',
               'const double T4 = T3 * T;', 'const double T5 = T4 * T;',
               'const double m = P / ({} * T);'.format(ru),
               '(void)m; (void)T2; (void)T3; (void)T4; (void)T5;']
     if any(x.plog or x.cheb for x in ir.rxns):
-        lines += ['const double logP = crs::log64(P);',
+        lines += ['const double logP = crs::log64_finite(P);',
                   'const double log10P = logP * 0.43429448190325182;',
                   '(void)log10P;']
     lines += ['const double log2T = logT * 1.4426950408889634;',

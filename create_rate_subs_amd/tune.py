@@ -46,7 +46,7 @@ def build_variant(mech, name, opts):
     return lib
 
 
-def run(mech, names, n=1 << 20, steps=10, seed=2001):
+def run(mech, names, n=1 << 20, steps=10, seed=2001, rounds=5):
     """Time the default library and each named variant; print JSON lines."""
     import torch
     from . import metrics, runtime, states
@@ -68,12 +68,16 @@ def run(mech, names, n=1 << 20, steps=10, seed=2001):
                             'libcrs_{}.so'.format(mech))
         libs.append((name, runtime.MechLib(path)))
     out = torch.empty_like(Yd)
-    for name, lib in libs:
-        for mode in ('fp64', 'mixed'):
-            if mode == 'mixed' and not lib.modes & 2:
-                continue
-            for _ in range(2):
-                lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+    runs = [(name, lib, mode) for name, lib in libs
+            for mode in ('fp64', 'mixed') if mode == 'fp64' or lib.modes & 2]
+    for name, lib, mode in runs:
+        for _ in range(2):
+            lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+    # rounds interleave the libraries so that clock drift over the run
+    # biases none of them; the median round is reported
+    times = {(name, mode): [] for name, _lib, mode in runs}
+    for _ in range(rounds):
+        for name, lib, mode in runs:
             torch.cuda.synchronize()
             t0 = torch.cuda.Event(enable_timing=True)
             t1 = torch.cuda.Event(enable_timing=True)
@@ -82,15 +86,18 @@ def run(mech, names, n=1 << 20, steps=10, seed=2001):
                 lib.eval_wdot(Td, Pd, Yd, mode, out=out)
             t1.record()
             torch.cuda.synchronize()
-            ms = t0.elapsed_time(t1) / steps
-            finite = torch.isfinite(out)
-            e = metrics.rel_err(out, ref, S, TAU)
-            e = torch.where(finite, e, torch.zeros_like(e))
-            print(json.dumps(dict(
-                mechanism=mech, variant=name, mode=mode, states=n, ms=ms,
-                states_per_s=n / (ms * 1e-3),
-                nonfinite_fraction=1.0 - finite.double().mean().item(),
-                max_err_vs_default_fp64=e.max().item(), tau=TAU)), flush=True)
+            times[(name, mode)].append(t0.elapsed_time(t1) / steps)
+    for name, lib, mode in runs:
+        lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+        ms = sorted(times[(name, mode)])[rounds // 2]
+        finite = torch.isfinite(out)
+        e = metrics.rel_err(out, ref, S, TAU)
+        e = torch.where(finite, e, torch.zeros_like(e))
+        print(json.dumps(dict(
+            mechanism=mech, variant=name, mode=mode, states=n, ms=ms,
+            ms_rounds=times[(name, mode)], states_per_s=n / (ms * 1e-3),
+            nonfinite_fraction=1.0 - finite.double().mean().item(),
+            max_err_vs_default_fp64=e.max().item(), tau=TAU)), flush=True)
 
 
 def main(argv=None):

@@ -28,7 +28,7 @@ def _coeffs_module():
 
 def _device_constants():
     text = open(os.path.join(CSRC, 'crs_common.h')).read()
-    body = text[text.index('double log64(double x)'):]
+    body = text[text.index('double log64_finite(double x)'):]
     body = body[:body.index('\n}\n')]
     return [float(v) for v in re.findall(r'(-?\d+\.\d+(?:e-?\d+)?)', body)]
 
