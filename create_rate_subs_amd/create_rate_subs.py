@@ -18,7 +18,7 @@ import argparse
 import os
 import sys
 
-from . import chem, utils
+from . import chem, mech_ct, utils
 from . import mech_interpret as mech
 
 __all__ = ['load_mechanism', 'create_rate_subs', 'main', 'langs']
@@ -48,10 +48,12 @@ def load_mechanism(mech_name, therm_name=None, last_spec=None, log=None):
     species lists turned into indices.
     """
     say = log if log is not None else (lambda msg: None)
-    if mech_name.endswith(('.cti', '.xml')):
-        raise NotImplementedError('Cantera input needs cantera, which is '
-                                  'not available; convert to Chemkin')
-    elems, specs, reacs = mech.read_mech(mech_name, therm_name)
+    # Cantera formats (create_rate_subs.py:466-471); YAML is read without
+    # cantera, .cti/.xml need cantera itself
+    if mech_name.endswith(('.cti', '.xml', '.yaml', '.yml')):
+        elems, specs, reacs = mech_ct.read_mech_ct(mech_name)
+    else:
+        elems, specs, reacs = mech.read_mech(mech_name, therm_name)
 
     idx = None
     if last_spec is not None:

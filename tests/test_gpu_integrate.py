@@ -8,7 +8,8 @@ against the oracle elsewhere), at steps small enough to stay finite:
 
 * RK4 on the GPU is bit-identical to the textbook RK4 combination done on
   the host with the stage derivatives the GPU's crs_dydt returns for the
-  host's own stage states (the combination kernel is not contracted);
+  host's own stage states (the combination kernel is not contracted), for
+  1, 2 and 3 steps;
 * the harness' captured HIP graph, replayed several times from the same
   initial state with eager work and synchronisations in between, gives the
   eager result every time;
@@ -73,6 +74,18 @@ def test_rk4_is_the_textbook_combination(name):
     assert np.isfinite(got).all()
     assert moved > 0.01
     np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize('steps', [1, 2])
+def test_short_runs_are_the_textbook_combination(steps):
+    """One and two steps (the step count's edge cases)."""
+    name = 'mech53'
+    P, y0 = _state(name, 300, 64)
+    lib = hip_lib(name)
+    Pd, yd = to_dev(P, y0)
+    ref = _rk4_host(lib, Pd, y0, H, steps)
+    lib.rk4(Pd, yd, 0.0, H, steps)
+    np.testing.assert_array_equal(to_host(yd), ref)
 
 
 def test_graph_replays_match_eager():
