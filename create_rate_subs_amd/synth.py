@@ -50,6 +50,9 @@ class MechConfig:
     frac_plog: float = 0.0
     frac_cheb: float = 0.0
     seed: int = 0
+    # species k with k % 3 == 1 get this T_mid instead of 1000 K (0: none),
+    # so that a mechanism has two T_mid groups
+    tmid_alt: float = 0.0
 
 
 # The three mechanisms the workloads are built on.
@@ -62,6 +65,9 @@ MECHS = {
     # pressure-dependent rate forms (PLOG, Chebyshev) beside the others
     'mech21pc': MechConfig('mech21pc', 21, 60, 0.80, 0.10, 3, frac_troe=0.05,
                            frac_plog=0.15, frac_cheb=0.15, seed=1005),
+    # two T_mid groups (1000 K and 1300 K): the per-group range masks
+    'mech17tm': MechConfig('mech17tm', 17, 50, 0.85, 0.12, 3, frac_troe=0.08,
+                           frac_lind=0.04, seed=1007, tmid_alt=1300.0),
 }
 
 # PLOG table pressures [atm] and Chebyshev validity box of the generator
@@ -89,8 +95,9 @@ def _pick_compositions(rng, count):
     return lattice[:count]
 
 
-def _fit_nasa(rng):
-    """Random smooth cp/R -> NASA-7 (lo, hi) coefficient lists."""
+def _fit_nasa(rng, tm=1000.0):
+    """Random smooth cp/R -> NASA-7 (lo, hi) coefficient lists, ranges
+    [300, tm] and [tm, 5000] K."""
     c = rng.uniform(0.5, 8.0)
     theta = rng.uniform(500.0, 3000.0)
     h298 = rng.normal(0.0, 150.0e3)           # J/mol
@@ -107,8 +114,8 @@ def _fit_nasa(rng):
         a, *_ = np.linalg.lstsq(V, cp(T), rcond=None)
         return [a[i] / 1000.0 ** i for i in range(5)]
 
-    lo = fit(300.0, 1000.0)
-    hi = fit(1000.0, 5000.0)
+    lo = fit(300.0, tm)
+    hi = fit(tm, 5000.0)
 
     def h_poly(a, T):      # H/R without a5
         return (a[0] * T + a[1] * T ** 2 / 2 + a[2] * T ** 3 / 3 +
@@ -118,7 +125,7 @@ def _fit_nasa(rng):
         return (a[0] * math.log(T) + a[1] * T + a[2] * T ** 2 / 2 +
                 a[3] * T ** 3 / 3 + a[4] * T ** 4 / 4)
 
-    T0, Tm = 298.15, 1000.0
+    T0, Tm = 298.15, tm
     lo5 = h298 / R - h_poly(lo, T0)
     lo6 = s298 / R - s_poly(lo, T0)
     hi5 = h_poly(lo, Tm) + lo5 - h_poly(hi, Tm)
@@ -130,14 +137,14 @@ def _fmt_coef(x):
     return '{:15.8E}'.format(x)
 
 
-def _thermo_entry(name, elem_counts, lo, hi):
+def _thermo_entry(name, elem_counts, lo, hi, tm=1000.0):
     """Four fixed-column NASA-7 lines (columns as read by read_thermo)."""
     elem = ''
     for el, n in elem_counts:
         elem += '{:<2}{:>3}'.format(el, n)
     elem = '{:<20}'.format(elem)
     l1 = '{:<18}{:<6}{}G{:>10.3f}{:>10.3f}{:>8.2f}'.format(
-        name, 'SYN', elem, 300.0, 5000.0, 1000.0)
+        name, 'SYN', elem, 300.0, 5000.0, tm)
     l1 = '{:<79}1'.format(l1)
     l2 = ''.join(_fmt_coef(x) for x in hi[0:5]) + '    2'
     l3 = ''.join(_fmt_coef(x) for x in (hi[5], hi[6], lo[0], lo[1], lo[2]))
@@ -284,9 +291,10 @@ def generate_mechanism(cfg):
     # thermo
     thermo = ['THERMO\n', '   300.000  1000.000  5000.000\n']
     for i, comp in enumerate(comps):
-        lo, hi = _fit_nasa(rng)
+        tm = cfg.tmid_alt if (cfg.tmid_alt and i % 3 == 1) else 1000.0
+        lo, hi = _fit_nasa(rng, tm)
         counts = [(el, n) for el, n in zip(_ELEMS, comp) if n]
-        thermo.append(_thermo_entry(names[i], counts, lo, hi))
+        thermo.append(_thermo_entry(names[i], counts, lo, hi, tm))
     # bath gas: drawn like the others from its own stream; it only ever
     # acts as a collider, so its thermo never enters an equilibrium constant
     lo, hi = _fit_nasa(np.random.default_rng(cfg.seed + 7919))

@@ -32,17 +32,21 @@ def variant_dir(mech, name):
 
 
 def build_variant(mech, name, opts):
-    """Emit and compile mechanism ``mech`` with emitter options ``opts``."""
+    """Emit and compile mechanism ``mech`` with emitter options ``opts``
+    (the key 'hipcc_flags', a list, is passed to hipcc instead)."""
     from .create_rate_subs import load_mechanism
     mpath, tpath = workloads.mechanism_files(mech)
     _e, specs, reacs, fwd = load_mechanism(mpath, tpath)
     out = variant_dir(mech, name)
+    opts = dict(opts)
+    flags = opts.pop('hipcc_flags', [])
     src = hip_emitter.write_hip(out, specs, reacs, name=mech, spec_order=fwd,
                                 opts=opts)
-    lib = build.compile_library(src, out)
+    lib = build.compile_library(src, out, extra_flags=flags)
     with open(os.path.join(out, 'opts.json'), 'w') as fh:
-        json.dump(dict(opts=opts, resolved=hip_emitter.resolve_opts(
-            len(specs), opts)), fh, indent=1)
+        json.dump(dict(opts=opts, hipcc_flags=flags,
+                       resolved=hip_emitter.resolve_opts(len(specs), opts)),
+                  fh, indent=1)
     return lib
 
 

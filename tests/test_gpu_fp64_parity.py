@@ -6,7 +6,9 @@ contraction, IEEE division on both sides); eval_rxn_rates and
 get_rxn_pres_mod may differ by the last bits of libm's exp/log (glibc on
 the host, the ROCm device library on the GPU), bounded at 1e-12 pointwise.
 The fused kernel is scored with the conditioning-aware relative error of
-create_rate_subs_amd.metrics against the oracle at <= 1e-12.
+create_rate_subs_amd.metrics against the oracle at <= 1e-12.  mech17tm has
+two T_mid groups (1000 K and 1300 K), so states on either side of either
+T_mid take each species' NASA range by its own T_mid.
 """
 
 import numpy as np
@@ -18,7 +20,8 @@ from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-MECHS = ['mech13', 'mech53', 'mech111', 'mech13co', 'mech21pc']
+MECHS = ['mech13', 'mech53', 'mech111', 'mech13co', 'mech21pc',
+         'mech17tm']
 # tau of the conditioned error: where |wdot_k| < tau * S_k, errors count
 # relative to tau * S_k.  Every rate constant is exp(x) with |x| up to
 # ~100 (ln A, E/T and the NASA polynomial of ln Kc), and two correct FP64
