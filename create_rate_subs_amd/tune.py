@@ -11,6 +11,8 @@ conditioned error of create_rate_subs_amd.metrics:
     python -m create_rate_subs_amd.tune build mech53 mfma '{"contraction": "mfma"}'
     # on the GPU: time the default and the named variants, 2^20 states
     python -m create_rate_subs_amd.tune run mech53 mfma
+    # on the GPU: the default library against the batch size
+    python -m create_rate_subs_amd.tune sweep mech53
 
 Variants live under _build/variants/<mechanism>/<name>/ with the options
 they were built from (opts.json).  One JSON line per (variant, mode).
@@ -104,12 +106,53 @@ def run(mech, names, n=1 << 20, steps=10, seed=2001, rounds=5):
             max_err_vs_default_fp64=e.max().item(), tau=TAU)), flush=True)
 
 
+def sweep(mech, exps=range(10, 23, 2), steps=10, seed=2001, rounds=3):
+    """States/s of the default library's fused kernels against the batch
+    size (2^k states for k in exps): where the launch fills the chip.
+    One JSON line per (size, mode)."""
+    import torch
+    from . import runtime, states
+    from .create_rate_subs import load_mechanism
+    mpath, tpath = workloads.mechanism_files(mech)
+    _e, specs, _r, _f = load_mechanism(mpath, tpath)
+    lib = runtime.load(build.lib_path(mech))
+    nmax = 1 << max(exps)
+    T, P, Y = states.random_states(len(specs), nmax, seed)
+    Td, Pd, Yd = (torch.from_numpy(a).cuda() for a in (T, P, Y))
+    for k in exps:
+        n = 1 << k
+        Tn, Pn, Yn = Td[:n], Pd[:n], Yd[:, :n].contiguous()
+        out = torch.empty_like(Yn)
+        for mode in ('fp64', 'mixed'):
+            if mode == 'mixed' and not lib.modes & 2:
+                continue
+            for _ in range(2):
+                lib.eval_wdot(Tn, Pn, Yn, mode, out=out)
+            best = []
+            for _ in range(rounds):
+                torch.cuda.synchronize()
+                t0 = torch.cuda.Event(enable_timing=True)
+                t1 = torch.cuda.Event(enable_timing=True)
+                t0.record()
+                for _ in range(steps):
+                    lib.eval_wdot(Tn, Pn, Yn, mode, out=out)
+                t1.record()
+                torch.cuda.synchronize()
+                best.append(t0.elapsed_time(t1) / steps)
+            ms = sorted(best)[rounds // 2]
+            print(json.dumps(dict(mechanism=mech, states=n, mode=mode, ms=ms,
+                                  states_per_s=n / (ms * 1e-3),
+                                  blocks=(n + 255) // 256)), flush=True)
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     if len(argv) >= 4 and argv[0] == 'build':
         print(build_variant(argv[1], argv[2], json.loads(argv[3])))
     elif len(argv) >= 2 and argv[0] == 'run':
         run(argv[1], argv[2:])
+    elif len(argv) >= 2 and argv[0] == 'sweep':
+        sweep(argv[1])
     else:
         print(__doc__)
         return 2
