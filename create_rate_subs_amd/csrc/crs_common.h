@@ -132,6 +132,44 @@ __device__ __forceinline__ double exp2_64(double t)
     return __builtin_ldexp(p, (int)n);
 }
 
+// exp2_64 / exp64 with q(r) by Estrin's scheme instead of Horner's: the
+// same coefficients, 3 more multiplies, a dependency chain of 5 operations
+// instead of 10 (1.20 ulp max, as Horner, by exact emulation in
+// exp64_coeffs.py).
+__device__ __forceinline__ double exp_q_estrin(double r)
+{
+    const double r2 = r * r;
+    const double r4 = r2 * r2;
+    const double r8 = r4 * r4;
+    const double p01 = __builtin_fma(0.16666666666666669, r, 0.5000000000000001);
+    const double p23 = __builtin_fma(0.008333333333330062, r, 0.04166666666662413);
+    const double p45 = __builtin_fma(0.00019841269863053618, r, 0.0013888888917213717);
+    const double p67 = __builtin_fma(2.7557268459997064e-06, r, 2.4801521295954376e-05);
+    const double p89 = __builtin_fma(2.510038549551032e-08, r, 2.7620088445409746e-07);
+    const double p03 = __builtin_fma(p23, r2, p01);
+    const double p47 = __builtin_fma(p67, r2, p45);
+    return __builtin_fma(p89, r8, __builtin_fma(p47, r4, p03));
+}
+
+__device__ __forceinline__ double exp2_64_estrin(double t)
+{
+    const double n = __builtin_rint(t);
+    const double r = (t - n) * 0.6931471805599453;
+    const double q = exp_q_estrin(r);
+    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
+__device__ __forceinline__ double exp64_estrin(double x)
+{
+    const double n = __builtin_rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-n, 0.6931471805599453, x);
+    r = __builtin_fma(-n, 2.3190468138462996e-17, r);
+    const double q = exp_q_estrin(r);
+    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
 // ln x in FP64 to about 2 ulp (max relative error 1.81 * 2^-53 measured by
 // exact emulation with a reciprocal seed 2^-23 off).  x = 2^e m with m in
 // [sqrt(1/2), sqrt(2)), f = m - 1 (exact), s = f / (2 + f) from the

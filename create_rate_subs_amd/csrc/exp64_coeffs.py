@@ -71,6 +71,41 @@ def emulate2(t, cs, hi):
     return rd(mp.ldexp(p, int(n)))
 
 
+def emulate2_estrin(t, cs, hi):
+    """crs::exp2_64_estrin: as emulate2 with q(r) by Estrin's scheme."""
+    def rd(v):
+        return mp.mpf(float(v))
+
+    def fma(a, b, c):
+        return rd(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+    a = cs[::-1]                          # a0 .. a9
+    n = mp.nint(t)
+    r = rd((mp.mpf(t) - n) * hi)
+    r2 = rd(r * r)
+    r4 = rd(r2 * r2)
+    r8 = rd(r4 * r4)
+    p01, p23 = fma(a[1], r, a[0]), fma(a[3], r, a[2])
+    p45, p67 = fma(a[5], r, a[4]), fma(a[7], r, a[6])
+    p89 = fma(a[9], r, a[8])
+    q = fma(p89, r8, fma(fma(p67, r2, p45), r4, fma(p23, r2, p01)))
+    p = fma(r, fma(r, q, 1.0), 1.0)
+    return rd(mp.ldexp(p, int(n)))
+
+
+def max_error2_estrin(samples=20000, seed=4):
+    """Max relative error of crs::exp2_64_estrin, units of 2^-53."""
+    cs = coefficients()
+    hi, _lo = ln2_split()
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        t = rng.uniform(-1000.0, 1000.0) if i % 2 else rng.uniform(-3.0, 3.0)
+        ex = mp.power(2, mp.mpf(t))
+        worst = max(worst, float(abs(emulate2_estrin(t, cs, hi) - ex) / ex))
+    return worst / 2.0 ** -53
+
+
 def max_error2(samples=20000, seed=2):
     """Max relative error of crs::exp2_64 against 2^t, units of 2^-53."""
     cs = coefficients()

@@ -682,6 +682,10 @@ def _indent(lines, n):
 #   defaults 0.941 / 0.667; without the LDS prefetch 1.024 / 0.774, with 1
 #   block instead of 3 0.944 / 0.681 (4-8 blocks within 1 %); without the
 #   NASA table 0.976 / 0.704; without conc_late 0.944 / 0.680;
+# * exp64 / exp2_64 with Estrin's scheme (exp_poly='estrin', dependency
+#   chain 5 instead of 10, 3 more multiplies): 0.920 -> 0.967 ms FP64 at 53
+#   species, 3.30 -> 3.49 ms at 111; the scheduler already interleaves the
+#   reactions' Horner chains, so the extra instructions only cost;
 # * the reaction-block literals as a wave-uniform LDS table
 #   (consts_rx='lds' / 'lds_fp64', _block_consts_to_lds) remove 2/3 of the
 #   scalar moves and nearly all scalar spills but cost as many LDS reads
@@ -1867,6 +1871,9 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                         opts.get('consts_prefetch', opts.get('lds_prefetch'))))
     if opts.get('lds_prefetch') and sources:
         lines = _prefetch_lane_lds(lines, sources, opts['lds_prefetch'])
+    if opts.get('exp_poly') == 'estrin':
+        lines = [l.replace('crs::exp2_64(', 'crs::exp2_64_estrin(')
+                  .replace('crs::exp64(', 'crs::exp64_estrin(') for l in lines]
     return lines
 
 

@@ -50,3 +50,12 @@ def test_exp2_64_shares_the_polynomial_and_error_bound():
     assert consts[0] == hi
     assert consts[1:11] == mod.coefficients()
     assert mod.max_error2(samples=3000, seed=8) <= 1.5
+
+
+def test_estrin_form_shares_coefficients_and_bound():
+    """exp_q_estrin holds the same ten coefficients (paired low/high power)
+    and the Estrin evaluation keeps the 1.5 ulp bound."""
+    mod = _coeffs_module()
+    consts = _device_constants('double exp_q_estrin(double r)')
+    assert sorted(consts) == sorted(mod.coefficients())
+    assert mod.max_error2_estrin(samples=3000, seed=9) <= 1.5
