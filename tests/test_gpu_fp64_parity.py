@@ -92,6 +92,13 @@ def test_wdot_fp64_matches_oracle(name, n):
     assert s['max'] <= 1e-12, s
 
 
+def test_wdot_fp64_matches_oracle_at_headline_size():
+    """The headline configuration itself: 53 species, 2^20 states, every
+    state against the oracle (the largest error of a sample grows with
+    its size, so the bound is checked on the full batch)."""
+    test_wdot_fp64_matches_oracle('mech53', 1 << 20)
+
+
 @pytest.mark.parametrize('name', ['mech53'])
 def test_wdot_subset(name):
     import torch
