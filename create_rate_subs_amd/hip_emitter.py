@@ -1629,8 +1629,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     expfn = 'crs::exp_mixed' if mixed else fexp
     # Arrhenius exponentials in base 2 (constants pre-scaled by log2 e):
     # exp2_mixed in the mixed kernel, exp2_64 in the FP64 one
-    arr2 = ('crs::exp2_mixed' if mixed else
-            'crs::exp2_64' if opts.get('exp') == 'crs' else None)
+    # mixed_exp='fp64' keeps the mixed kernel's Arrhenius exponentials in
+    # FP64 (to measure how much of its error they carry)
+    arr2 = ('crs::exp2_mixed' if mixed and opts.get('mixed_exp') != 'fp64'
+            else 'crs::exp2_64' if opts.get('exp') == 'crs' or mixed else None)
     tmids = []
     for x in ir.rxns:
         for tm, _lo, _hi in x.kc_groups:
@@ -1657,6 +1659,8 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
             lines += dydt_lines
     hg = opts.get('kc') == 'species_hg'
     species_kc = opts.get('kc') in ('species', 'species_hg')
+    # mixed kernel: FP32 falloff / third-body blocks unless mixed_pm='fp64'
+    mixed_pm32 = mixed and opts.get('mixed_pm', 'fp32') == 'fp32'
     nasa_lds = opts.get('nasa') == 'lds'
     if nasa_lds:
         nasa_lds = max(1, int(opts.get('nasa_prefetch', 2))) if opts.get(
@@ -1725,7 +1729,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
         if hg and _hg_ok(x):
             pm = ''
             if x.pd_index is not None:
-                inner += (_mixed_pm_lines(x, K, xf) if mixed else
+                inner += (_mixed_pm_lines(x, K, xf) if mixed_pm32 else
                           _fast_pm_lines(x, K, xf, arr2))
                 pm = ' * pm'
             inner += _hg_net_rate(x, gref, pm)
@@ -1793,7 +1797,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                         _conc_prod(x.prod)))
         q = '(f - r)' if x.rev else 'f'
         if x.pd_index is not None:
-            inner += (_mixed_pm_lines(x, K, xf) if mixed else
+            inner += (_mixed_pm_lines(x, K, xf) if mixed_pm32 else
                       _fast_pm_lines(x, K, xf, arr2))
             q = q + ' * pm'
         inner.append('const double q = {};'.format(q))
