@@ -1593,15 +1593,18 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     """
     nsp = ir.nsp
     ru = _lit(emitted(fmt_e8(chem.RU)))
+    # preload: the kernel loaded T and P before its LDS table fill
+    tp = (['const double T = T_pre;', 'const double P = P_pre;']
+          if opts.get('preload') else None)
     if dydt:
         # the reference's state vector: y[0] = T, y[1 + k] = Y_k
-        lines = ['const double T = Y[i];', 'const double P = P_[i];',
-                 'double y[{}];'.format(max(nsp - 1, 1))]
+        lines = (tp or ['const double T = Y[i];', 'const double P = P_[i];']) + [
+            'double y[{}];'.format(max(nsp - 1, 1))]
         lines += ['y[{k}] = Y[{r} * ld + i];'.format(k=k, r=k + 1)
                   for k in range(nsp - 1)]
     else:
-        lines = ['const double T = T_[i];', 'const double P = P_[i];',
-                 'double y[{}];'.format(max(nsp - 1, 1))]
+        lines = (tp or ['const double T = T_[i];', 'const double P = P_[i];']) + [
+            'double y[{}];'.format(max(nsp - 1, 1))]
         lines += ['y[{k}] = Y[{k} * ld + i];'.format(k=k)
                   for k in range(nsp - 1)]
     # conc_late: the T-only work (basis, G_k) runs while the mass fractions
@@ -1881,6 +1884,29 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     return lines
 
 
+def _idx_kernel_tail(fill, call, preload):
+    """Body of a fused kernel over a batch (idx == nullptr) or a subset:
+    the block's LDS table fill, then one state per thread.  With preload the
+    thread's T and P are loaded before the fill, so their latency overlaps
+    the fill's and its barrier instead of following them."""
+    if not (preload and fill):
+        return (fill +
+                '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + '
+                'threadIdx.x;\n'
+                '    if (t >= m) return;\n'
+                '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
+                '    if (i < 0 || i >= ld) return;\n'
+                '    ' + call.format(pre='') + '\n}\n\n')
+    return ('    const int64_t t = (int64_t)blockIdx.x * blockDim.x + '
+            'threadIdx.x;\n'
+            '    const int64_t i = t < m ? (idx ? (int64_t)idx[t] : t) : -1;\n'
+            '    const bool live = i >= 0 && i < ld;\n'
+            '    const double T_pre = live ? T_[i] : 0.0;\n'
+            '    const double P_pre = live ? P_[i] : 0.0;\n' + fill +
+            '    if (!live) return;\n'
+            '    ' + call.format(pre=', T_pre, P_pre') + '\n}\n\n')
+
+
 def emit_hip_source(ir, opts=None):
     """Full HIP source text for mechanism ``ir``."""
     opts = resolve_opts(ir.nsp, opts)
@@ -2029,16 +2055,20 @@ def emit_hip_source(ir, opts=None):
         return lines, nasa_fill + fill
 
     # ---- fused FP64 per-state body ------------------------------------
+    # preload needs a fill to overlap with: the NASA table's
+    preload = (bool(opts.get('preload')) and opts['fused'] == 'fast' and
+               bool(nasa_fill))
     if opts['fused'] == 'fast':
-        f64_lines, f64_fill = fused_state('f64', opts)
+        f64_lines, f64_fill = fused_state('f64', dict(opts, preload=preload))
     else:
         f64_lines, f64_fill = _fused_f64_lines(ir, K, opts), nasa_fill
+    pre_arg = ', double T_pre, double P_pre' if preload else ''
     body.append(
         '__device__ __forceinline__ void wdot_f64_state(\n'
         '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
-        '    int64_t ld, int64_t i)\n{\n' +
+        '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
         _indent(f64_lines, 4) + '}\n\n')
     # one kernel serves the full batch (idx == nullptr) and subsets
     body.append(
@@ -2047,30 +2077,40 @@ def emit_hip_source(ir, opts=None):
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
         '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
-        f64_fill +
-        '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
-        '    if (t >= m) return;\n'
-        '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
-        '    if (i < 0 || i >= ld) return;\n'
-        '    wdot_f64_state(K, T_, P_, Y, W, ld, i);\n}\n\n')
+        _idx_kernel_tail(f64_fill,
+                         'wdot_f64_state(K, T_, P_, Y, W, ld, i{pre});',
+                         preload))
 
     # ---- constant-pressure dydt (FP64), the caller of the path -------
     dydt_lines, dydt_fill = fused_state(
-        'dydt', dict(opts, contraction='sparse'), dydt=True)
+        'dydt', dict(opts, contraction='sparse', preload=preload), dydt=True)
     body.append(
         '__device__ __forceinline__ void dydt_f64_state(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
-        '    int64_t ld, int64_t i)\n{\n' +
+        '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
         _indent(dydt_lines, 4) + '}\n\n')
+    if preload and dydt_fill:
+        dydt_tail = (
+            '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
+            'threadIdx.x;\n'
+            '    const bool live = i < n;\n'
+            '    const double T_pre = live ? Y[i] : 0.0;\n'
+            '    const double P_pre = live ? P_[i] : 0.0;\n' + dydt_fill +
+            '    if (!live) return;\n'
+            '    dydt_f64_state(K, P_, Y, W, ld, i, T_pre, P_pre);\n}\n\n')
+    else:
+        dydt_tail = (
+            dydt_fill +
+            '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
+            'threadIdx.x;\n'
+            '    if (i >= n) return;\n'
+            '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
     body.append(
         '__global__ __launch_bounds__(crs::kBlock) void k_dydt_f64(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
-        '    int64_t ld)\n{\n' + dydt_fill +
-        '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
-        '    if (i >= n) return;\n'
-        '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
+        '    int64_t ld)\n{\n' + dydt_tail)
 
     # ---- RK4 stage combination for the batched integrator ------------
     body.append(
@@ -2111,8 +2151,11 @@ def emit_hip_source(ir, opts=None):
         '        t_ign[i] = t_new;\n}\n\n')
 
     # ---- fused mixed-precision per-state body -------------------------
+    mfma_mixed = opts.get('contraction') == 'mfma'
     if opts.get('mixed', True):
-        mixed_lines, mixed_fill = fused_state('mixed', opts, mixed=True)
+        mixed_lines, mixed_fill = fused_state(
+            'mixed', dict(opts, preload=preload and not mfma_mixed),
+            mixed=True)
     if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
         rx = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
         nst, nch, vals = _mfma_nu_table(ir, rx)
@@ -2149,7 +2192,7 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
-            '    int64_t ld, int64_t i)\n{\n' +
+            '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
             _indent(mixed_lines, 4) + '}\n\n')
         body.append(
             '__global__ __launch_bounds__(crs::kBlock) void k_wdot_mixed(\n'
@@ -2157,12 +2200,9 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
             '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
-            mixed_fill +
-            '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
-            '    if (t >= m) return;\n'
-            '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
-            '    if (i < 0 || i >= ld) return;\n'
-            '    wdot_mixed_state(K, T_, P_, Y, W, ld, i);\n}\n\n')
+            _idx_kernel_tail(mixed_fill,
+                             'wdot_mixed_state(K, T_, P_, Y, W, ld, i{pre});',
+                             preload))
 
     header = (
         '// Generated by create_rate_subs_amd.hip_emitter for mechanism '
