@@ -37,8 +37,10 @@ def parse():
     ap.add_argument('--workload', default='w1_53')
     ap.add_argument('--states', type=int, default=0,
                     help='override the workload\'s state count')
-    ap.add_argument('--cpu-sample', type=int, default=1 << 16,
-                    help='states in the bounded CPU-baseline sample')
+    ap.add_argument('--cpu-sample', type=int, default=1 << 20,
+                    help='states in the bounded CPU-baseline sample (the '
+                         'whole 2^20-state workload by default: about 20 s '
+                         'of CPU work over the timed passes)')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
@@ -75,10 +77,11 @@ def cpu_baseline(name, T, P, Y, sample, threads):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     return dict(value=m / t, unit='states/s', cores=threads, kind='port',
-                sample='{} of the workload\'s states, {} OpenMP threads, '
-                       'median of 3 after 1 warm-up ({:.3f} s each); host {}'
-                       .format(m, threads, t, platform.processor() or
-                               platform.machine()))
+                sample='{} of the workload\'s {} states, {} OpenMP threads, '
+                       'median of 3 after 1 warm-up ({:.3f} s each, {:.1f} s '
+                       'of CPU work timed); host {}'
+                       .format(m, T.size, threads, t, 3 * t * threads,
+                               platform.processor() or platform.machine()))
 
 
 def measured_traffic(kernel, workload, n):
