@@ -92,6 +92,24 @@ def test_wdot_fp64_matches_oracle(name, n):
     assert s['max'] <= 1e-12, s
 
 
+@pytest.mark.parametrize('name', ['mech13co', 'mech53', 'mech17tm'])
+def test_wdot_fp64_near_equilibrium(name):
+    """Element-potential equilibrium states perturbed by 1e-4: forward and
+    reverse rates nearly cancel in every reversible reaction, which the
+    fused kernel forms as one FMA on H products (prod H_r - K prod H_p)
+    and the oracle as f - r; the conditioned error keeps the same bound."""
+    specs, reacs, _ = mechanism(name)
+    T, P, Y = states.equilibrium_states(specs, 16384, 43, perturb=1e-4)
+    ref = oracle(name).eval_stages(T, P, Y)
+    lib = hip_lib(name)
+    W = to_host(lib.eval_wdot(*to_dev(T, P, Y)))
+    S = metrics.gross_scale(specs, reacs, ref['fwd'], ref['rev'], ref['pm'])
+    s = metrics.summarize(metrics.rel_err(W, ref['wdot'], S, TAU))
+    cancel = np.abs(ref['wdot']) / np.maximum(S, 1e-300)
+    print(name, 'near-eq', s, 'median |w| / S', float(np.median(cancel)))
+    assert s['max'] <= 1e-12, s
+
+
 def test_wdot_fp64_matches_oracle_at_headline_size():
     """The headline configuration itself: 53 species, 2^20 states, every
     state against the oracle (the largest error of a sample grows with
