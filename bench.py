@@ -325,9 +325,10 @@ def surrogate_section(args, torch, lib, specs, reacs):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: every box here has one MI355X '
-              'and this path shards trivially over states (no exchange); '
-              'multi-GPU runs are not part of this build.', file=sys.stderr)
+        print('bench.py measures one GPU only: the path shards trivially '
+              'over states (no exchange between GPUs), so N GPUs run N '
+              'independent copies of the one-GPU workload; no multi-GPU '
+              'run is made.', file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -10,7 +10,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line(
-        'markers', 'gpu: needs a real MI355X (run with -m gpu on the GPU box)')
+        'markers', 'gpu: needs a real MI355X (select with -m gpu)')
 
 
 def pytest_collection_modifyitems(config, items):
