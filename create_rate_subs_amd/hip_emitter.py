@@ -1178,10 +1178,15 @@ def _mixed_pm_lines(x, K, kf_x):
     if x.thd:
         out.append('const double pm = (double)({});'.format(_eff_expr_f(x)))
         return out
-    if x.pdep_sp is None:
-        out.append('const float Mf = {};'.format(_eff_expr_f(x)))
+    # a single species as collider: its concentration can be far below the
+    # FP32 range (or zero), so Pr and log10 Pr stay FP64 there, clamped as
+    # the reference clamps (fmax(Pr, 1e-300))
+    named = x.pdep_sp is not None
+    if named:
+        out.append('const double Md = C[{}];'.format(x.pdep_sp))
     else:
-        out.append('const float Mf = (float)C[{}];'.format(x.pdep_sp))
+        out.append('const float Mf = {};'.format(_eff_expr_f(x)))
+    coll = 'Md' if named else '(double)Mf'
     # every exponent here is base 2 (the mixed kernel's Arrhenius form)
     if x.low is not None:
         l0, x0 = _fast_arr(x.low, K, 'k0', 'crs::exp2_mixed', log2=True)
@@ -1197,11 +1202,14 @@ def _mixed_pm_lines(x, K, kf_x):
         den, xden = 'kf', kf_x
     if xnum is not None and xden is not None:
         out.append('const double xd = {} - {};'.format(xnum, xden))
-        out.append('const double Pr = (double)Mf * crs::exp2_mixed(xd);')
+        out.append('const double Pr = {} * crs::exp2_mixed(xd);'.format(coll))
         lpr = ('crs::log10f_hw(Mf) + (float)(xd * 0.30102999566398120)')
     else:
-        out.append('const double Pr = {} * (double)Mf / {};'.format(num, den))
+        out.append('const double Pr = {} * {} / {};'.format(num, coll, den))
         lpr = '(float)log10(fmax(Pr, 1.0e-300))'
+    if named:
+        lpr = ('(float)(crs::log64_finite(fmax(Pr, 1.0e-300)) * '
+               '0.4342944819032518)')
     ratio = ''
     if x.falloff_low:
         ratio = ' * (Pr * crs::rcp64(1.0 + Pr))'

@@ -53,6 +53,13 @@ class MechConfig:
     # species k with k % 3 == 1 get this T_mid instead of 1000 K (0: none),
     # so that a mechanism has two T_mid groups
     tmid_alt: float = 0.0
+    # the remaining rate-form branches (from a separate random stream, so
+    # the draws above are unchanged): every 2nd Troe reaction without T**,
+    # every 2nd SRI one with d and e, every 3rd falloff reaction with a
+    # single species as collider, every 2nd Lindemann one chemically
+    # activated (HIGH instead of LOW), every 5th reversible elementary one
+    # with explicit reverse parameters (REV)
+    form_variants: bool = False
 
 
 # The three mechanisms the workloads are built on.
@@ -68,6 +75,11 @@ MECHS = {
     # two T_mid groups (1000 K and 1300 K): the per-group range masks
     'mech17tm': MechConfig('mech17tm', 17, 50, 0.85, 0.12, 3, frac_troe=0.08,
                            frac_lind=0.04, seed=1007, tmid_alt=1300.0),
+    # the rarer rate forms: 3-parameter Troe, 5-parameter SRI, a species
+    # as falloff collider, chemically activated (HIGH), explicit REV
+    'mech19fx': MechConfig('mech19fx', 19, 80, 0.80, 0.10, 3, frac_troe=0.10,
+                           frac_lind=0.10, frac_sri=0.08, seed=1009,
+                           form_variants=True),
 }
 
 # PLOG table pressures [atm] and Chebyshev validity box of the generator
@@ -166,6 +178,9 @@ class _Rxn:
     eff: list = field(default_factory=list)
     plog: list = field(default_factory=list)   # (P atm, A, b, Ea) rows
     cheb: list = field(default_factory=list)   # [n_temp][n_pres] coeffs
+    high: tuple = ()            # chemically activated: HIGH instead of LOW
+    coll: int = -1              # falloff collider species (-1: M)
+    rev_par: tuple = ()         # explicit reverse Arrhenius (REV)
 
 
 def _side_text(terms, names):
@@ -181,6 +196,41 @@ def _arrhenius(rng):
     b = round(rng.uniform(-2.0, 3.0), 3)
     Ea = round(rng.uniform(0.0, 60.0) * 1000.0, 3)    # cal/mol
     return A, b, Ea
+
+
+def _form_variants(rxns, nsp, rng):
+    """Turn some reactions into the rarer rate forms (see MechConfig)."""
+    n_troe = n_sri = n_fall = n_lind = n_rev = 0
+    for r in rxns:
+        if r.kind == 'troe':
+            n_troe += 1
+            if n_troe % 2 == 0:
+                r.troe = r.troe[:3]
+        if r.kind == 'sri':
+            n_sri += 1
+            if n_sri % 2 == 0:
+                r.sri = r.sri + (float('%.6g' % rng.uniform(0.5, 2.0)),
+                                 float('%.4f' % rng.uniform(-0.5, 0.5)))
+        if r.kind in ('troe', 'lind', 'sri'):
+            n_fall += 1
+            if n_fall % 3 == 0:
+                inside = {k for k, _nu in r.reac + r.prod}
+                free = [k for k in range(nsp) if k not in inside]
+                r.coll = int(free[int(rng.integers(len(free)))])
+                r.eff = []
+        if r.kind == 'lind':
+            n_lind += 1
+            if n_lind % 2 == 0:
+                # main line = low-pressure limit, HIGH = high-pressure one
+                A, b, Ea = r.low
+                r.arr, r.low = (A, b, Ea), ()
+                r.high = (float('%.6E' % (A * 10.0 ** -rng.uniform(2.0, 6.0))),
+                          round(rng.uniform(-2.0, 3.0), 3),
+                          round(rng.uniform(0.0, 60.0) * 1000.0, 3))
+        if r.kind == 'elem' and r.rev:
+            n_rev += 1
+            if n_rev % 5 == 0:
+                r.rev_par = _arrhenius(rng)
 
 
 def generate_mechanism(cfg):
@@ -287,6 +337,8 @@ def generate_mechanism(cfg):
                      float('%.6g' % rng.uniform(100.0, 2000.0)),
                      float('%.6g' % rng.uniform(100.0, 2000.0)))
         rxns.append(r)
+    if cfg.form_variants:
+        _form_variants(rxns, nsp, np.random.default_rng(cfg.seed + 31337))
 
     # thermo
     thermo = ['THERMO\n', '   300.000  1000.000  5000.000\n']
@@ -314,20 +366,26 @@ def generate_mechanism(cfg):
             lhs += ' + M'
             rhs += ' + M'
         elif r.kind in ('troe', 'lind', 'sri'):
-            lhs += ' (+M)'
-            rhs += ' (+M)'
+            coll = 'M' if r.coll < 0 else names[r.coll]
+            lhs += ' (+{})'.format(coll)
+            rhs += ' (+{})'.format(coll)
         arrow = '<=>' if r.rev else '=>'
         A, b, Ea = r.arr
         mech.append('{} {} {}   {:.6E} {:.3f} {:.3f}\n'.format(
             lhs, arrow, rhs, A, b, Ea))
         if r.low:
             mech.append('    LOW / {:.6E} {:.3f} {:.3f} /\n'.format(*r.low))
+        if r.high:
+            mech.append('    HIGH / {:.6E} {:.3f} {:.3f} /\n'.format(*r.high))
+        if r.rev_par:
+            mech.append('    REV / {:.6E} {:.3f} {:.3f} /\n'.format(
+                *r.rev_par))
         if r.troe:
-            mech.append('    TROE / {} {} {} {} /\n'.format(
-                *('%.6g' % x for x in r.troe)))
+            mech.append('    TROE / {} /\n'.format(
+                ' '.join('%.6g' % x for x in r.troe)))
         if r.sri:
-            mech.append('    SRI / {} {} {} /\n'.format(
-                *('%.6g' % x for x in r.sri)))
+            mech.append('    SRI / {} /\n'.format(
+                ' '.join('%.6g' % x for x in r.sri)))
         if r.eff:
             mech.append('    ' + ' '.join('{}/{:.2f}/'.format(names[k], e)
                                           for k, e in r.eff) + '\n')

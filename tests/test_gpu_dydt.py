@@ -23,7 +23,8 @@ from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 TAU = 5e-2
-LIBS = ['mech13', 'mech53', 'mech111', 'mech13co', 'mech21pc', 'mech17tm']
+LIBS = ['mech13', 'mech53', 'mech111', 'mech13co', 'mech21pc', 'mech17tm',
+        'mech19fx']
 
 
 def _inputs(name, n, seed, rows=None, ld=None):

@@ -40,7 +40,7 @@ def _score(name, T, P, Y):
 
 
 @pytest.mark.parametrize('name', ['mech13', 'mech53', 'mech111', 'mech21pc',
-                                  'mech17tm'])
+                                  'mech17tm', 'mech19fx'])
 def test_mixed_random_states(name):
     specs, _r, _f = mechanism(name)
     T, P, Y = states.random_states(len(specs), 65536, 41)

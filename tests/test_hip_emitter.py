@@ -48,7 +48,8 @@ def test_lane_lds_rows_and_prefetch(name, rows, ref):
         assert ('crs_nasa_lds' in b) == (rows == 'sGl'), fn
 
 
-@pytest.mark.parametrize('name', ['mech53', 'mech111', 'mech21pc', 'mech17tm'])
+@pytest.mark.parametrize('name', ['mech53', 'mech111', 'mech21pc', 'mech17tm',
+                                  'mech19fx'])
 def test_fused_fp64_bodies_call_no_library_transcendental(name):
     _text, bodies = _bodies(name)
     for fn in ('wdot_f64_state', 'dydt_f64_state'):

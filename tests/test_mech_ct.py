@@ -24,7 +24,8 @@ from create_rate_subs_amd import chem, mech_ct, workloads
 from create_rate_subs_amd import mech_interpret as mi
 from create_rate_subs_amd.create_rate_subs import load_mechanism
 
-MECHS = ['mech21pc', 'mech53', 'mech111', 'mech13co', 'mech17tm']
+MECHS = ['mech21pc', 'mech53', 'mech111', 'mech13co', 'mech17tm',
+         'mech19fx']
 
 CAL = 4184.0                 # J/kcal: J/kmol per cal/mol
 SI = dict(length='m', quantity='kmol', E='J/kmol', Ef=1.0, conc=1.0,

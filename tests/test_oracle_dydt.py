@@ -32,7 +32,7 @@ from oracle.build import CFLAGS, HERE as ORACLE_DIR, Oracle
 
 from _helpers import mechanism, oracle
 
-MECHS = ['mech13', 'mech53', 'mech111', 'mech17tm']
+MECHS = ['mech13', 'mech53', 'mech111', 'mech17tm', 'mech19fx']
 
 
 def _state_vector(specs, n, seed):
