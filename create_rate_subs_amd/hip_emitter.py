@@ -369,7 +369,8 @@ def _conc_prod(terms, C='C'):
         if is_integer(nu):
             s += '{}[{}] * '.format(C, k) * int(nu)
         else:
-            s += 'pow({}[{}], {}) * '.format(C, k, _lit(emitted(py2_str(nu))))
+            s += 'crs::pow64({}[{}], {}) * '.format(
+                C, k, _lit(emitted(py2_str(nu))))
     return s
 
 

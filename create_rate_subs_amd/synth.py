@@ -58,7 +58,8 @@ class MechConfig:
     # every 2nd SRI one with d and e, every 3rd falloff reaction with a
     # single species as collider, every 2nd Lindemann one chemically
     # activated (HIGH instead of LOW), every 5th reversible elementary one
-    # with explicit reverse parameters (REV)
+    # with explicit reverse parameters (REV), every 3rd irreversible
+    # elementary one with all coefficients halved (non-integer orders)
     form_variants: bool = False
 
 
@@ -186,7 +187,13 @@ class _Rxn:
 def _side_text(terms, names):
     out = []
     for sp, nu in terms:
-        out.append(('2' if nu == 2 else '') + names[sp])
+        if nu == 1:
+            coef = ''
+        elif nu == int(nu):
+            coef = str(int(nu))
+        else:
+            coef = '{:g}'.format(nu)
+        out.append(coef + names[sp])
     return ' + '.join(out)
 
 
@@ -200,7 +207,7 @@ def _arrhenius(rng):
 
 def _form_variants(rxns, nsp, rng):
     """Turn some reactions into the rarer rate forms (see MechConfig)."""
-    n_troe = n_sri = n_fall = n_lind = n_rev = 0
+    n_troe = n_sri = n_fall = n_lind = n_rev = n_irr = 0
     for r in rxns:
         if r.kind == 'troe':
             n_troe += 1
@@ -231,6 +238,12 @@ def _form_variants(rxns, nsp, rng):
             n_rev += 1
             if n_rev % 5 == 0:
                 r.rev_par = _arrhenius(rng)
+        if r.kind == 'elem' and not r.rev:
+            n_irr += 1
+            if n_irr % 3 == 0:
+                # still balanced: both sides scaled by 1/2
+                r.reac = [(k, nu / 2.0) for k, nu in r.reac]
+                r.prod = [(k, nu / 2.0) for k, nu in r.prod]
 
 
 def generate_mechanism(cfg):
