@@ -692,11 +692,26 @@ def _indent(lines, n):
 #   scalar moves and nearly all scalar spills but cost as many LDS reads
 #   and AGPR copies: 4 % faster FP64 before the changes above, 1-5 %
 #   slower after them; not the default.
+# * two waves per SIMD (waves_per_simd=2): the kernel needs 200 VGPRs and
+#   no AGPRs, so only the LDS rows (115.7 KB per block at 53 species) held
+#   it to one.  39 G rows per wave (2 blocks of 4 waves in 160 KB, no NASA
+#   table), the G_k of the 14 species that retire first in registers
+#   (g_pick='retire'), eval_conc first: ~100 VGPRs spill (110 scratch
+#   accesses per wave in 25k instructions), yet the second wave hides the
+#   latencies one wave could not.  Interleaved at 53 species, FP64 /
+#   mixed ms: defaults 0.940 / 0.666 against 0.795-0.844 / 0.597-0.642 over
+#   four runs of 38-39 rows; 35 rows with the NASA table 0.830 / 0.611;
+#   three waves (25 rows, 520 VGPRs spilled) 1.546 / 1.508.
 # 'auto' picks the H/G form with G in LDS when every species fits the LDS
-# budget (78 rows), otherwise species Kc with 78 concentration rows in LDS.
+# budget (78 rows), otherwise species Kc with 78 concentration rows in LDS;
+# waves_per_simd='auto' takes two waves where 40-56 species would fill the
+# LDS at one (the range measured: 53 species).
 DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
                     fused='fast', kc='auto', exp='crs', order='retire',
-                    nasa='auto')
+                    nasa='auto', waves_per_simd='auto')
+
+# per-lane G rows per wave at two waves per SIMD (two 4-wave blocks per CU)
+_ROWS_2W = (80 * 1024 - 512) // 2048
 
 
 def resolve_opts(nsp, opts=None):
@@ -705,12 +720,26 @@ def resolve_opts(nsp, opts=None):
     LDS budget at one wave per SIMD: 160 KB per CU for 4 waves.  Per-lane
     rows take 512 B per wave each; the NASA coefficient table (option
     nasa='lds') takes 128 B per species per block.  nasa='auto' takes the
-    table where the H/G form fits beside it.
+    table where the H/G form fits beside it.  waves_per_simd='auto' gives
+    40-56 species two waves per SIMD instead: 39 rows per wave, the other
+    G_k in registers, no table.
     """
     o = dict(DEFAULT_OPTS, **(opts or {}))
 
     def rows(table):
         return min(78, (160 * 1024 - table - 1024) // 2048)
+    if o.get('waves_per_simd') == 'auto':
+        two = (_ROWS_2W < nsp <= 56 and 'lds_species' not in o and
+               o.get('kc') in ('auto', 'species_hg'))
+        o['waves_per_simd'] = 2 if two else None
+        if two:
+            o['lds_species'] = _ROWS_2W
+            o['kc'] = 'species_hg'
+            o.setdefault('g_store', 'lane_lds')
+            o.setdefault('g_pick', 'retire')
+            o.setdefault('lds_prefetch', 1)
+            if o.get('nasa') == 'auto':
+                o['nasa'] = None
     if o.get('nasa') == 'auto':
         fits = nsp <= rows(128 * nsp) and o.get('kc') in ('auto', 'species_hg')
         o['nasa'] = 'lds' if fits else None
@@ -1691,14 +1720,25 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     g_lane = hg and opts.get('g_store') == 'lane_lds'
     # LDS budget: one wave per SIMD leaves 40 KB per wave, 80 FP64 rows
     n_gl = min(nsp, opts.get('lds_species', 78)) if g_lane else 0
-    gref = (lambda k: 'gl[{} * 64]'.format(k) if k < n_gl else
+    rxns = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
+    # which species get an LDS row when not all fit: the first n_gl by
+    # index, or (g_pick='retire') the n_gl whose last reaction comes latest,
+    # so the G_k left in registers are the ones that die first
+    if opts.get('g_pick') == 'retire' and n_gl < nsp:
+        end = {k: p for p, x in enumerate(rxns) for k, _nu in x.spec}
+        by_end = sorted(range(nsp), key=lambda k: (-end.get(k, -1), k))
+        gl_slot = {k: s for s, k in enumerate(sorted(by_end[:n_gl]))}
+    else:
+        gl_slot = {k: k for k in range(n_gl)}
+    gref = (lambda k: 'gl[{} * 64]'.format(gl_slot[k]) if k in gl_slot else
             'G{}'.format(k))
     if hg and g_lane:
         lines.append('__shared__ double sGl[crs::kBlock / 64][{}][64];'
                      .format(n_gl))
         lines.append('CRS_LDS double *const gb = (CRS_LDS double *)'
                      '&sGl[threadIdx.x >> 6][0][threadIdx.x & 63];')
-        lines += ['gb[{k} * 64] = G{k};'.format(k=k) for k in range(n_gl)]
+        lines += ['gb[{s} * 64] = G{k};'.format(s=s, k=k)
+                  for k, s in sorted(gl_slot.items())]
     if late:
         lines += conc + dydt_lines
     if hg:
@@ -1706,7 +1746,6 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
         lines += ['const double H{k} = C[{k}] * crs::rcp64(G{k});'.format(k=k)
                   for k in range(nsp)]
     rx_start = len(lines)
-    rxns = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
     last = _last_touch(ir, rxns)
     seen = [False] * nsp
     mfma = mixed and opts.get('contraction') == 'mfma'
@@ -1918,11 +1957,22 @@ def _idx_kernel_tail(fill, call, preload):
 
 def emit_hip_source(ir, opts=None):
     """Full HIP source text for mechanism ``ir``."""
-    opts = resolve_opts(ir.nsp, opts)
+    raw = dict(opts or {})
+    opts = resolve_opts(ir.nsp, raw)
+    # the CONP dydt kernel keeps one wave per SIMD unless asked otherwise:
+    # its cp and h terms add live registers that two waves would spill
+    dopts = (resolve_opts(ir.nsp, dict(raw, waves_per_simd=None))
+             if raw.get('waves_per_simd', 'auto') == 'auto' else opts)
     K = _Consts(opts['consts'])
     nsp, nr, nrev, npd = ir.nsp, ir.nr, ir.nrev, ir.npd
     ru = _lit(emitted(fmt_e8(chem.RU)))
     body = []
+    # fused kernels: waves per SIMD the register allocation must leave room
+    # for (HIP's second launch bound)
+    lb = (', {}'.format(int(opts['waves_per_simd']))
+          if opts.get('waves_per_simd') else '')
+    dlb = (', {}'.format(int(dopts['waves_per_simd']))
+           if dopts.get('waves_per_simd') else '')
 
     # ---- stage kernel: eval_conc -------------------------------------
     lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
@@ -2027,7 +2077,7 @@ def emit_hip_source(ir, opts=None):
 
     # ---- NASA coefficient table in LDS (option nasa='lds') -------------
     nasa_fill = ''
-    if opts.get('nasa') == 'lds':
+    if 'lds' in (opts.get('nasa'), dopts.get('nasa')):
         vals = _nasa_table_values(ir)
         body.append('__device__ const double crs_nasa_tab[{}] = {{\n'.format(
             len(vals)) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
@@ -2045,9 +2095,10 @@ def emit_hip_source(ir, opts=None):
         (the FP64 ones) the body's reaction-block literals live in an LDS
         table (declared here, filled by the kernel)."""
         mode = opts.get('consts_rx')
+        nfill = nasa_fill if o.get('nasa') == 'lds' else ''
         if mode not in ('lds', 'lds_fp64') or (mode == 'lds_fp64' and
                                                 name == 'mixed'):
-            return _fused_fast_lines(ir, K, o, **kw), nasa_fill
+            return _fused_fast_lines(ir, K, o, **kw), nfill
         tab = []
         lines = _fused_fast_lines(ir, K, o, ktab=tab,
                                   ktab_name='crs_kt_' + name, **kw)
@@ -2061,16 +2112,17 @@ def emit_hip_source(ir, opts=None):
         fill = ('    for (int q = threadIdx.x; q < {n}; q += blockDim.x)\n'
                 '        crs_kt_{name}[q] = crs_kt_{name}_src[q];\n'
                 '    __syncthreads();\n'.format(n=n, name=name))
-        return lines, nasa_fill + fill
+        return lines, nfill + fill
 
     # ---- fused FP64 per-state body ------------------------------------
     # preload needs a fill to overlap with: the NASA table's
     preload = (bool(opts.get('preload')) and opts['fused'] == 'fast' and
-               bool(nasa_fill))
+               opts.get('nasa') == 'lds' and dopts.get('nasa') == 'lds')
     if opts['fused'] == 'fast':
         f64_lines, f64_fill = fused_state('f64', dict(opts, preload=preload))
     else:
-        f64_lines, f64_fill = _fused_f64_lines(ir, K, opts), nasa_fill
+        f64_lines, f64_fill = _fused_f64_lines(ir, K, opts), (
+            nasa_fill if opts.get('nasa') == 'lds' else '')
     pre_arg = ', double T_pre, double P_pre' if preload else ''
     body.append(
         '__device__ __forceinline__ void wdot_f64_state(\n'
@@ -2081,7 +2133,7 @@ def emit_hip_source(ir, opts=None):
         _indent(f64_lines, 4) + '}\n\n')
     # one kernel serves the full batch (idx == nullptr) and subsets
     body.append(
-        '__global__ __launch_bounds__(crs::kBlock) void k_wdot_f64(\n'
+        '__global__ __launch_bounds__(crs::kBlock' + lb + ') void k_wdot_f64(\n'
         '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
@@ -2092,7 +2144,7 @@ def emit_hip_source(ir, opts=None):
 
     # ---- constant-pressure dydt (FP64), the caller of the path -------
     dydt_lines, dydt_fill = fused_state(
-        'dydt', dict(opts, contraction='sparse', preload=preload), dydt=True)
+        'dydt', dict(dopts, contraction='sparse', preload=preload), dydt=True)
     body.append(
         '__device__ __forceinline__ void dydt_f64_state(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
@@ -2116,7 +2168,7 @@ def emit_hip_source(ir, opts=None):
             '    if (i >= n) return;\n'
             '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
     body.append(
-        '__global__ __launch_bounds__(crs::kBlock) void k_dydt_f64(\n'
+        '__global__ __launch_bounds__(crs::kBlock' + dlb + ') void k_dydt_f64(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
         '    int64_t ld)\n{\n' + dydt_tail)
@@ -2181,7 +2233,7 @@ def emit_hip_source(ir, opts=None):
             '    int64_t ld, int64_t i, bool live)\n{\n' +
             _indent(mixed_lines, 4) + '}\n\n')
         body.append(
-            '__global__ __launch_bounds__(crs::kBlock) void k_wdot_mixed(\n'
+            '__global__ __launch_bounds__(crs::kBlock' + lb + ') void k_wdot_mixed(\n'
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
@@ -2204,7 +2256,7 @@ def emit_hip_source(ir, opts=None):
             '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
             _indent(mixed_lines, 4) + '}\n\n')
         body.append(
-            '__global__ __launch_bounds__(crs::kBlock) void k_wdot_mixed(\n'
+            '__global__ __launch_bounds__(crs::kBlock' + lb + ') void k_wdot_mixed(\n'
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'

@@ -33,10 +33,19 @@ def _bodies(name, opts=None):
     return text, out
 
 
-@pytest.mark.parametrize('name,rows,ref', [('mech53', 'sGl', 'gl'),
-                                           ('mech111', 'sCl', 'cl')])
-def test_lane_lds_rows_and_prefetch(name, rows, ref):
-    _text, bodies = _bodies(name)
+@pytest.mark.parametrize('name,rows,ref,table,waves', [
+    ('mech21pc', 'sGl', 'gl', True, None),
+    ('mech53', 'sGl', 'gl', False, 2),
+    ('mech111', 'sCl', 'cl', False, None)])
+def test_lane_lds_rows_and_prefetch(name, rows, ref, table, waves):
+    text, bodies = _bodies(name)
+    # 40-56 species: two waves per SIMD with 39 G rows per wave
+    bound = '__launch_bounds__(crs::kBlock{}) void k_wdot_f64('.format(
+        ', {}'.format(waves) if waves else '')
+    assert bound in text
+    if waves:
+        assert '__shared__ double sGl[crs::kBlock / 64][39][64];' in text
+        assert '__launch_bounds__(crs::kBlock) void k_dydt_f64(' in text
     for fn, b in bodies.items():
         assert '__shared__ double {}['.format(rows) in b, fn
         # every block's reads come from a prefetch group outside the block
@@ -45,7 +54,23 @@ def test_lane_lds_rows_and_prefetch(name, rows, ref):
         assert not any(re.search(r'\b{}\['.format(ref), blk) for blk in blocks)
         assert re.search(r'const double {}\d+_\d+ = {}p\d+\['.format(ref, ref), b)
         # the NASA table rides along where the H/G rows leave room for it
-        assert ('crs_nasa_lds' in b) == (rows == 'sGl'), fn
+        # (the dydt kernel stays at one wave per SIMD, where they do)
+        want = table or (waves is not None and fn == 'dydt_f64_state')
+        assert ('crs_nasa_lds' in b) == want, fn
+
+
+def test_two_wave_layout_keeps_late_retiring_species_in_lds():
+    """With 39 of 53 G rows in LDS, the 14 G_k left in registers are those
+    of the species whose last reaction comes first in the emitted order."""
+    ir = _ir('mech53')
+    _text, bodies = _bodies('mech53')
+    b = bodies['wdot_f64_state']
+    stored = {int(k) for k in re.findall(r'gb\[\d+ \* 64\] = G(\d+);', b)}
+    assert len(stored) == 39
+    rxns = hip_emitter._retiring_order(ir)
+    end = {k: p for p, x in enumerate(rxns) for k, _nu in x.spec}
+    in_regs = set(range(ir.nsp)) - stored
+    assert max(end.get(k, -1) for k in in_regs) <= min(end[k] for k in stored)
 
 
 @pytest.mark.parametrize('name', ['mech53', 'mech111', 'mech21pc', 'mech17tm',
