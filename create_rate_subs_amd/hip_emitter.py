@@ -737,6 +737,7 @@ def resolve_opts(nsp, opts=None):
             o['kc'] = 'species_hg'
             o.setdefault('g_store', 'lane_lds')
             o.setdefault('g_pick', 'retire')
+            o.setdefault('g_reload', True)
             o.setdefault('lds_prefetch', 1)
             if o.get('nasa') == 'auto':
                 o['nasa'] = None
@@ -1742,8 +1743,16 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     if late:
         lines += conc + dydt_lines
     if hg:
-        # H_k = C_k / G_k; from here on C_k is G_k H_k (no third array)
-        lines += ['const double H{k} = C[{k}] * crs::rcp64(G{k});'.format(k=k)
+        # H_k = C_k / G_k; from here on C_k is G_k H_k (no third array).
+        # g_reload: G_k of the LDS species read back through an opaque
+        # pointer, so its register dies at the store instead of living
+        # across eval_conc
+        reload = g_lane and opts.get('g_reload')
+        if reload:
+            lines.append('CRS_LDS double *const gh = crs::opaque_lds(gb);')
+        lines += ['const double H{k} = C[{k}] * crs::rcp64({g});'.format(
+                      k=k, g='gh[{} * 64]'.format(gl_slot[k])
+                      if reload and k in gl_slot else 'G{}'.format(k))
                   for k in range(nsp)]
     rx_start = len(lines)
     last = _last_touch(ir, rxns)
