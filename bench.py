@@ -32,8 +32,9 @@ TAU = 5e-2
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=50)
+    # enough warm-up launches (tens of ms) for the clocks to settle
+    ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--workload', default='w1_53')
     ap.add_argument('--states', type=int, default=0,
                     help='override the workload\'s state count')

@@ -696,12 +696,14 @@ def _indent(lines, n):
 #   no AGPRs, so only the LDS rows (115.7 KB per block at 53 species) held
 #   it to one.  39 G rows per wave (2 blocks of 4 waves in 160 KB, no NASA
 #   table), the G_k of the 14 species that retire first in registers
-#   (g_pick='retire'), eval_conc first: ~100 VGPRs spill (110 scratch
-#   accesses per wave in 25k instructions), yet the second wave hides the
+#   (g_pick='retire'), the LDS species' G_k read back for H_k
+#   (g_reload), rows read 1 block ahead: 138 VGPRs spill (144 scratch
+#   accesses per wave in 24k instructions), yet the second wave hides the
 #   latencies one wave could not.  Interleaved at 53 species, FP64 /
-#   mixed ms: defaults 0.940 / 0.666 against 0.795-0.844 / 0.597-0.642 over
-#   four runs of 38-39 rows; 35 rows with the NASA table 0.830 / 0.611;
-#   three waves (25 rows, 520 VGPRs spilled) 1.546 / 1.508.
+#   mixed ms: one wave 0.919 / 0.649 against 0.757-0.780 / 0.568-0.579;
+#   36 rows with the NASA table 0.822 / 0.656; three waves (25 rows, 520
+#   VGPRs spilled) 1.546 / 1.508; dydt at two waves 1.21 against 1.13 (it
+#   stays at one).  At 111 species two waves are 1.7x slower.
 # 'auto' picks the H/G form with G in LDS when every species fits the LDS
 # budget (78 rows), otherwise species Kc with 78 concentration rows in LDS;
 # waves_per_simd='auto' takes two waves where 40-56 species would fill the

@@ -74,11 +74,23 @@ def run(mech, names, n=1 << 20, steps=10, seed=2001, rounds=5):
                             'libcrs_{}.so'.format(mech))
         libs.append((name, runtime.MechLib(path)))
     out = torch.empty_like(Yd)
+    # the CONP right-hand side on the same states (row 0 = T, rows
+    # 1..NSP-1 = Y), scored against the default library's
+    yd = torch.cat([Td[None], Yd[:-1]]).contiguous()
+    dref = base.dydt(Pd, yd)
+    dout = torch.empty_like(yd)
+
+    def call(lib, mode):
+        if mode == 'dydt':
+            lib.dydt(Pd, yd, out=dout)
+        else:
+            lib.eval_wdot(Td, Pd, Yd, mode, out=out)
     runs = [(name, lib, mode) for name, lib in libs
-            for mode in ('fp64', 'mixed') if mode == 'fp64' or lib.modes & 2]
+            for mode in ('fp64', 'mixed', 'dydt')
+            if mode != 'mixed' or lib.modes & 2]
     for name, lib, mode in runs:
         for _ in range(2):
-            lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+            call(lib, mode)
     # rounds interleave the libraries so that clock drift over the run
     # biases none of them; the median round is reported
     times = {(name, mode): [] for name, _lib, mode in runs}
@@ -89,16 +101,22 @@ def run(mech, names, n=1 << 20, steps=10, seed=2001, rounds=5):
             t1 = torch.cuda.Event(enable_timing=True)
             t0.record()
             for _ in range(steps):
-                lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+                call(lib, mode)
             t1.record()
             torch.cuda.synchronize()
             times[(name, mode)].append(t0.elapsed_time(t1) / steps)
     for name, lib, mode in runs:
-        lib.eval_wdot(Td, Pd, Yd, mode, out=out)
+        call(lib, mode)
         ms = sorted(times[(name, mode)])[rounds // 2]
-        finite = torch.isfinite(out)
-        e = metrics.rel_err(out, ref, S, TAU)
-        e = torch.where(finite, e, torch.zeros_like(e))
+        if mode == 'dydt':
+            finite = torch.isfinite(dout)
+            e = ((dout - dref).abs() /
+                 dref.abs().clamp_min(1e-300)).where(finite, 0.0)
+            e = e.where(dref.abs() > 1e-30, 0.0)
+        else:
+            finite = torch.isfinite(out)
+            e = metrics.rel_err(out, ref, S, TAU)
+            e = torch.where(finite, e, torch.zeros_like(e))
         print(json.dumps(dict(
             mechanism=mech, variant=name, mode=mode, states=n, ms=ms,
             ms_rounds=times[(name, mode)], states_per_s=n / (ms * 1e-3),
