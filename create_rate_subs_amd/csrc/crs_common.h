@@ -65,9 +65,9 @@ inline int launched(const char *fn)
     return CRS_OK;
 }
 
-inline dim3 grid_for(int64_t n)
+inline dim3 grid_for(int64_t n, int block = kBlock)
 {
-    return dim3((unsigned)((n + kBlock - 1) / kBlock));
+    return dim3((unsigned)((n + block - 1) / block));
 }
 
 // 1/x to within an ulp: hardware reciprocal estimate refined by two

@@ -1315,7 +1315,7 @@ def _concentrations_to_lds(lines, nsp):
     return out
 
 
-def _concentrations_to_lane_lds(lines, nsp, nl):
+def _concentrations_to_lane_lds(lines, nsp, nl, block='crs::kBlock'):
     """Park the concentrations of species 0..nl-1 in LDS, one slot per lane.
 
     Each wave owns an [nl][64] FP64 array (a species row is 512 bytes, so
@@ -1337,8 +1337,8 @@ def _concentrations_to_lane_lds(lines, nsp, nl):
     published = False
     for l in lines:
         if not published and l == _CONC_DONE:
-            out.append('__shared__ double sCl[crs::kBlock / 64][{}][64];'
-                       .format(max(len(slot), 1)))
+            out.append('__shared__ double sCl[{} / 64][{}][64];'
+                       .format(block, max(len(slot), 1)))
             out.append('CRS_LDS double *const cb = (CRS_LDS double *)'
                        '&sCl[threadIdx.x >> 6][0][threadIdx.x & 63];')
             out += ['cb[{s} * 64] = C[{k}];'.format(s=s, k=k)
@@ -1736,8 +1736,8 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     gref = (lambda k: 'gl[{} * 64]'.format(gl_slot[k]) if k in gl_slot else
             'G{}'.format(k))
     if hg and g_lane:
-        lines.append('__shared__ double sGl[crs::kBlock / 64][{}][64];'
-                     .format(n_gl))
+        lines.append('__shared__ double sGl[{} / 64][{}][64];'
+                     .format(opts.get('block_expr', 'crs::kBlock'), n_gl))
         lines.append('CRS_LDS double *const gb = (CRS_LDS double *)'
                      '&sGl[threadIdx.x >> 6][0][threadIdx.x & 63];')
         lines += ['gb[{s} * 64] = G{k};'.format(s=s, k=k)
@@ -1762,7 +1762,8 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     mfma = mixed and opts.get('contraction') == 'mfma'
     if mfma:
         nst = (nsp + 15) // 16
-        lines.append('__shared__ float sQ[crs::kBlock / 64][{}][64];'.format(
+        lines.append('__shared__ float sQ[{} / 64][{}][64];'.format(
+            opts.get('block_expr', 'crs::kBlock'),
             _MF_CHUNK))
         lines.append('floatx4 Dq[{}][4];'.format(nst))
         lines.append('#pragma unroll')
@@ -1924,7 +1925,8 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
         lines = _concentrations_to_lds(lines, ir.nsp)
     elif opts.get('c_store') == 'lane_lds':
         lines = _concentrations_to_lane_lds(
-            lines, ir.nsp, min(ir.nsp, opts.get('lds_species', 78)))
+            lines, ir.nsp, min(ir.nsp, opts.get('lds_species', 78)),
+            opts.get('block_expr', 'crs::kBlock'))
         sources.append(('cb', 'cl', True))
     if ktab is not None:
         first = next(p for p, l in enumerate(lines)
@@ -1984,6 +1986,11 @@ def emit_hip_source(ir, opts=None):
           if opts.get('waves_per_simd') else '')
     dlb = (', {}'.format(int(dopts['waves_per_simd']))
            if dopts.get('waves_per_simd') else '')
+    # threads per block of k_wdot_f64 / k_wdot_mixed (a multiple of 64)
+    wblock = int(opts.get('block') or 256)
+    if wblock % 64 or not 64 <= wblock <= 1024:
+        raise ValueError('block must be a multiple of 64 in [64, 1024]')
+    body.append('constexpr int kWdotBlock = {};\n\n'.format(wblock))
 
     # ---- stage kernel: eval_conc -------------------------------------
     lines = ['const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;',
@@ -2130,7 +2137,8 @@ def emit_hip_source(ir, opts=None):
     preload = (bool(opts.get('preload')) and opts['fused'] == 'fast' and
                opts.get('nasa') == 'lds' and dopts.get('nasa') == 'lds')
     if opts['fused'] == 'fast':
-        f64_lines, f64_fill = fused_state('f64', dict(opts, preload=preload))
+        f64_lines, f64_fill = fused_state('f64', dict(
+            opts, preload=preload, block_expr='kWdotBlock'))
     else:
         f64_lines, f64_fill = _fused_f64_lines(ir, K, opts), (
             nasa_fill if opts.get('nasa') == 'lds' else '')
@@ -2144,7 +2152,7 @@ def emit_hip_source(ir, opts=None):
         _indent(f64_lines, 4) + '}\n\n')
     # one kernel serves the full batch (idx == nullptr) and subsets
     body.append(
-        '__global__ __launch_bounds__(crs::kBlock' + lb + ') void k_wdot_f64(\n'
+        '__global__ __launch_bounds__(kWdotBlock' + lb + ') void k_wdot_f64(\n'
         '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
@@ -2226,7 +2234,8 @@ def emit_hip_source(ir, opts=None):
     mfma_mixed = opts.get('contraction') == 'mfma'
     if opts.get('mixed', True):
         mixed_lines, mixed_fill = fused_state(
-            'mixed', dict(opts, preload=preload and not mfma_mixed),
+            'mixed', dict(opts, preload=preload and not mfma_mixed,
+                          block_expr='kWdotBlock'),
             mixed=True)
     if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
         rx = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
@@ -2244,7 +2253,7 @@ def emit_hip_source(ir, opts=None):
             '    int64_t ld, int64_t i, bool live)\n{\n' +
             _indent(mixed_lines, 4) + '}\n\n')
         body.append(
-            '__global__ __launch_bounds__(crs::kBlock' + lb + ') void k_wdot_mixed(\n'
+            '__global__ __launch_bounds__(kWdotBlock' + lb + ') void k_wdot_mixed(\n'
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
@@ -2267,7 +2276,7 @@ def emit_hip_source(ir, opts=None):
             '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
             _indent(mixed_lines, 4) + '}\n\n')
         body.append(
-            '__global__ __launch_bounds__(crs::kBlock' + lb + ') void k_wdot_mixed(\n'
+            '__global__ __launch_bounds__(kWdotBlock' + lb + ') void k_wdot_mixed(\n'
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
@@ -2296,13 +2305,13 @@ def _abi_text(ir, mixed=True):
     modes = '(1 << CRS_MODE_FP64)' + (' | (1 << CRS_MODE_MIXED)' if mixed else '')
     if mixed:
         launch_full = '''    if (mode == CRS_MODE_MIXED)
-        k_wdot_mixed<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, P, Y, wdot,
-                                                          nullptr, n, ld);
+        k_wdot_mixed<<<crs::grid_for(n, kWdotBlock), kWdotBlock, 0, s>>>(
+            K, T, P, Y, wdot, nullptr, n, ld);
     else
 '''
         launch_sub = '''    if (mode == CRS_MODE_MIXED)
-        k_wdot_mixed<<<crs::grid_for(m), crs::kBlock, 0, s>>>(K, T, P, Y, wdot,
-                                                          idx, m, ld);
+        k_wdot_mixed<<<crs::grid_for(m, kWdotBlock), kWdotBlock, 0, s>>>(
+            K, T, P, Y, wdot, idx, m, ld);
     else
 '''
         mixed_ok = 'mode == CRS_MODE_MIXED'
@@ -2452,8 +2461,8 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
-{launch_full}    k_wdot_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, T, P, Y, wdot,
-                                                        nullptr, n, ld);
+{launch_full}    k_wdot_f64<<<crs::grid_for(n, kWdotBlock), kWdotBlock, 0, s>>>(
+        K, T, P, Y, wdot, nullptr, n, ld);
     return crs::launched(fn);
 }}
 
@@ -2532,8 +2541,8 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
-{launch_sub}    k_wdot_f64<<<crs::grid_for(m), crs::kBlock, 0, s>>>(K, T, P, Y, wdot, idx,
-                                                        m, ld);
+{launch_sub}    k_wdot_f64<<<crs::grid_for(m, kWdotBlock), kWdotBlock, 0, s>>>(
+        K, T, P, Y, wdot, idx, m, ld);
     return crs::launched(fn);
 }}
 

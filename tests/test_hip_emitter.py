@@ -40,11 +40,12 @@ def _bodies(name, opts=None):
 def test_lane_lds_rows_and_prefetch(name, rows, ref, table, waves):
     text, bodies = _bodies(name)
     # 40-56 species: two waves per SIMD with 39 G rows per wave
-    bound = '__launch_bounds__(crs::kBlock{}) void k_wdot_f64('.format(
+    bound = '__launch_bounds__(kWdotBlock{}) void k_wdot_f64('.format(
         ', {}'.format(waves) if waves else '')
     assert bound in text
+    assert 'constexpr int kWdotBlock = 256;' in text
     if waves:
-        assert '__shared__ double sGl[crs::kBlock / 64][39][64];' in text
+        assert '__shared__ double sGl[kWdotBlock / 64][39][64];' in text
         assert '__launch_bounds__(crs::kBlock) void k_dydt_f64(' in text
     for fn, b in bodies.items():
         assert '__shared__ double {}['.format(rows) in b, fn
