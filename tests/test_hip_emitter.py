@@ -137,3 +137,23 @@ def test_constant_table_keeps_every_value():
     nums = [float(v) for l in restored for v in re.findall(num, l)]
     orig = [float(v) for l in lines[1:3] for v in re.findall(num, l)]
     assert nums == orig
+
+
+@pytest.mark.parametrize('nsp,two', [(13, False), (39, False), (40, True),
+                                     (53, True), (56, True), (57, False),
+                                     (111, False)])
+def test_two_wave_choice(nsp, two):
+    """waves_per_simd='auto': two waves with 39 G rows from 40 to 56
+    species, one wave (all rows, or concentration rows) otherwise."""
+    o = hip_emitter.resolve_opts(nsp)
+    assert (o['waves_per_simd'] == 2) == two
+    if two:
+        assert o['lds_species'] == 39 and o['kc'] == 'species_hg'
+        assert o['nasa'] is None and o['g_reload'] and o['lds_prefetch'] == 1
+    else:
+        assert o['waves_per_simd'] is None
+    # an explicit row count or wave count is left alone
+    assert hip_emitter.resolve_opts(nsp, {'lds_species': 30})[
+        'waves_per_simd'] is None
+    assert hip_emitter.resolve_opts(nsp, {'waves_per_simd': None})[
+        'waves_per_simd'] is None
