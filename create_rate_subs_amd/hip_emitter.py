@@ -731,8 +731,12 @@ def resolve_opts(nsp, opts=None):
     def rows(table):
         return min(78, (160 * 1024 - table - 1024) // 2048)
     if o.get('waves_per_simd') == 'auto':
+        # only the per-lane G-row layout, and no NASA table (it would not
+        # leave room for a second block)
         two = (_ROWS_2W < nsp <= 56 and 'lds_species' not in o and
-               o.get('kc') in ('auto', 'species_hg'))
+               o.get('kc') in ('auto', 'species_hg') and
+               o.get('g_store', 'lane_lds') == 'lane_lds' and
+               o.get('nasa') != 'lds')
         o['waves_per_simd'] = 2 if two else None
         if two:
             o['lds_species'] = _ROWS_2W

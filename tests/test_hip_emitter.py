@@ -157,3 +157,6 @@ def test_two_wave_choice(nsp, two):
         'waves_per_simd'] is None
     assert hip_emitter.resolve_opts(nsp, {'waves_per_simd': None})[
         'waves_per_simd'] is None
+    # so is a layout the two-wave rows do not fit beside
+    for opts in ({'nasa': 'lds'}, {'g_store': 'lds'}, {'kc': 'species'}):
+        assert hip_emitter.resolve_opts(nsp, opts)['waves_per_simd'] is None
