@@ -293,6 +293,11 @@ def mp_state(specs, reacs, T, P, Y, dps=50):
                     a, b, c = (mp.mpf(x) for x in r.sri_par[0:3])
                     X = 1 / (1 + mp.log10(Pr) ** 2)
                     F = (a * mp.exp(-b / T) + mp.exp(-T / c)) ** X
+                    # d T^e, kept where the reference keeps it: only when
+                    # d != 1 and e != 0 (rate_subs.py:1033-1037)
+                    if (len(r.sri_par) == 5 and r.sri_par[3] != 1.0 and
+                            r.sri_par[4] != 0.0):
+                        F *= mp.mpf(r.sri_par[3]) * T ** mp.mpf(r.sri_par[4])
                 net *= F * (Pr / (1 + Pr) if r.low else 1 / (1 + Pr))
         for s, v in zip(r.reac, r.reac_nu):
             wdot[s] -= v * net
