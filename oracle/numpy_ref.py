@@ -304,3 +304,32 @@ def mp_state(specs, reacs, T, P, Y, dps=50):
         for s, v in zip(r.prod, r.prod_nu):
             wdot[s] += v * net
     return wdot
+
+
+def mp_dydt(specs, reacs, T, P, Y, dps=50):
+    """Constant-pressure right-hand side of one state in mpmath
+    (rate_subs.py:1829-1946): [dT/dt, dY_0/dt, ..., dY_NSP-2/dt], with the
+    mass-specific cp_k = RU/W_k (a0 + a1 T + ... + a4 T^4) and
+    h_k = RU T/W_k (a0 + a1 T/2 + ... + a4 T^4/5 + a5/T) of each species'
+    NASA range; also returns the mass-average cp and the density."""
+    import mpmath as mp
+    w = mp_state(specs, reacs, T, P, Y, dps)
+    T = mp.mpf(T)
+    P = mp.mpf(P)
+    nsp = len(specs)
+    y = [mp.mpf(float(v)) for v in Y[:nsp - 1]]
+    y.append(1 - mp.fsum(y))
+    W = [mp.mpf(sp.mw) for sp in specs]
+    ru = mp.mpf(RU)
+    rho = P / (ru * T * mp.fsum(y[k] / W[k] for k in range(nsp)))
+    cp, h = [], []
+    for k, sp in enumerate(specs):
+        a = [mp.mpf(float(x)) for x in (sp.lo if T <= sp.Trange[1] else sp.hi)]
+        cp.append(ru / W[k] * (a[0] + T * (a[1] + T * (a[2] + T * (a[3] +
+                                                              T * a[4])))))
+        h.append(ru * T / W[k] * (a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (
+            a[3] / 4 + T * a[4] / 5))) + a[5] / T))
+    cp_avg = mp.fsum(y[k] * cp[k] for k in range(nsp))
+    dT = -mp.fsum(w[k] * h[k] * W[k] for k in range(nsp)) / (rho * cp_avg)
+    dY = [w[k] * W[k] / rho for k in range(nsp - 1)]
+    return [dT] + dY, h, cp_avg, rho

@@ -1,14 +1,18 @@
 """Write tests/golden/wdot_kat.json: known-answer wdot vectors.
 
-For three synthetic mechanisms that between them hold every rate form of
-the path (mech13: elementary, three-body, Troe, Lindemann; mech21pc: PLOG
-and Chebyshev beside them; mech19fx: 3-parameter Troe, 5-parameter SRI, a
-named collider, chemically activated reactions, split REV reactions and
-non-integer orders), eight seeded states each, wdot evaluated in mpmath at
-50 digits from the closed forms (oracle/numpy_ref.mp_state, which restates
-the reference's definitions independently of both the C oracle and the
+For four synthetic mechanisms that between them hold every rate form of
+the path (mech13: elementary, three-body, Troe, Lindemann; mech13co: the
+same with a reactive species last, so the implied species' terms are
+non-zero; mech21pc: PLOG and Chebyshev beside them; mech19fx: 3-parameter
+Troe, 5-parameter SRI, a named collider, chemically activated reactions,
+split REV reactions and non-integer orders), eight seeded states each,
+wdot and the constant-pressure dydt evaluated in mpmath at 50 digits from
+the closed forms (oracle/numpy_ref.mp_state / mp_dydt, which restate the
+reference's definitions independently of both the C oracle and the
 kernels) and rounded to double.  S, the gross scale of the conditioned
-error metric, comes from the NumPy evaluator.
+error metric, comes from the NumPy evaluator; the dydt rows carry it
+through their scalings (dY_k/dt: S_k W_k / rho; dT/dt:
+sum_k |h_k W_k| S_k / (rho cp)).
 
     python tests/golden/make_wdot_kat.py
 
@@ -30,7 +34,7 @@ from create_rate_subs_amd import states, workloads  # noqa: E402
 from create_rate_subs_amd.create_rate_subs import load_mechanism  # noqa: E402
 from oracle import numpy_ref  # noqa: E402
 
-MECHS = ('mech13', 'mech21pc', 'mech19fx')
+MECHS = ('mech13', 'mech13co', 'mech21pc', 'mech19fx')
 COUNT = 8
 SEED = 41
 OUT = os.path.join(HERE, 'wdot_kat.json')
@@ -44,11 +48,22 @@ def make(name):
     ref = numpy_ref.evaluate(specs, reacs, T, P, Y)
     S = numpy_ref.gross_scale(specs, reacs, ref['fwd'], ref['rev'],
                               ref['pm'])
+    S = np.asarray(S)
     wdot = np.array([[float(v) for v in numpy_ref.mp_state(
         specs, reacs, T[i], P[i], Y[:, i])] for i in range(COUNT)]).T
+    mw = np.array([sp.mw for sp in specs])
+    dydt, dscale = [], []
+    for i in range(COUNT):
+        d, h, cp, rho = numpy_ref.mp_dydt(specs, reacs, T[i], P[i], Y[:, i])
+        dydt.append([float(v) for v in d])
+        h = np.array([float(v) for v in h])
+        cp, rho = float(cp), float(rho)
+        trow = (np.abs(h * mw) * S[:, i]).sum() / (rho * abs(cp))
+        dscale.append([trow] + list(S[:-1, i] * mw[:-1] / rho))
     return dict(nsp=len(specs), states=COUNT, seed=SEED, T=T.tolist(),
                 P=P.tolist(), Y=Y.tolist(), wdot=wdot.tolist(),
-                S=np.asarray(S).tolist())
+                S=S.tolist(), dydt=np.array(dydt).T.tolist(),
+                dydt_scale=np.array(dscale).T.tolist())
 
 
 def main():

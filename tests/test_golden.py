@@ -1,8 +1,9 @@
 """Known-answer vectors (tests/golden/wdot_kat.json, written by
-tests/golden/make_wdot_kat.py): wdot of three mechanisms that hold every
-rate form of the path, eight seeded states each, evaluated in mpmath at 50
-digits from the closed forms.  The C oracle (CPU) and the HIP kernels (GPU)
-must reproduce them under the conditioned error metric.
+tests/golden/make_wdot_kat.py): wdot and the constant-pressure dydt of four
+mechanisms that hold every rate form of the path, eight seeded states
+each, evaluated in mpmath at 50 digits from the closed forms.  The C oracle
+(CPU) and the HIP kernels (GPU) must reproduce them under the conditioned
+error metric.
 """
 
 import json
@@ -25,13 +26,25 @@ NAMES = sorted(KAT)
 # reduced-coordinate constants printed to 9 significant digits
 # (rate_subs.py:138-223, {:.8e}), which moves their rates by up to ~1e-8
 # from the unrounded textbook value; every other form agrees to rounding.
-TOL = {'mech13': 1e-11, 'mech19fx': 1e-11, 'mech21pc': 2e-8}
+TOL = {'mech13': 1e-11, 'mech13co': 1e-11, 'mech19fx': 1e-11,
+       'mech21pc': 2e-8}
 
 
 def _case(name):
     d = KAT[name]
     return (np.array(d['T']), np.array(d['P']), np.array(d['Y']),
             np.array(d['wdot']), np.array(d['S']))
+
+
+def _dydt_case(name):
+    """y in the reference's layout (row 0 = T, rows 1..NSP-1 = Y), P, the
+    expected dydt and its scale."""
+    d = KAT[name]
+    T, P, Y = (np.array(d[k]) for k in ('T', 'P', 'Y'))
+    y = np.empty((d['nsp'], T.size))
+    y[0] = T
+    y[1:] = Y[:d['nsp'] - 1]
+    return y, P, np.array(d['dydt']), np.array(d['dydt_scale'])
 
 
 @pytest.mark.parametrize('name', NAMES)
@@ -51,6 +64,25 @@ def test_oracle_matches_golden(name):
     T, P, Y, W, S = _case(name)
     got = oracle(name).eval_stages(T, P, Y)['wdot']
     e = metrics.rel_err(got, W, S, 1e-3)
+    assert e.max() <= TOL[name], e.max()
+
+
+@pytest.mark.parametrize('name', NAMES)
+def test_oracle_dydt_matches_golden(name):
+    y, P, D, DS = _dydt_case(name)
+    got = oracle(name).dydt(P, y)
+    e = metrics.rel_err(got, D, DS, 1e-3)
+    assert e.max() <= TOL[name], e.max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', NAMES)
+def test_hip_dydt_matches_golden(name):
+    y, P, D, DS = _dydt_case(name)
+    Pd, yd = to_dev(P, y)
+    got = to_host(hip_lib(name).dydt(Pd, yd))
+    e = metrics.rel_err(got, D, DS, 1e-3)
+    print(name, 'dydt vs golden', e.max())
     assert e.max() <= TOL[name], e.max()
 
 
