@@ -8,12 +8,15 @@ error metric.
 
 import json
 import os
+import re
+import subprocess
 
 import numpy as np
 import pytest
 
 from create_rate_subs_amd import metrics
-from oracle import numpy_ref
+from oracle import emit_c, numpy_ref
+from oracle.build import CFLAGS, HERE as ORACLE_DIR, Oracle
 
 from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
 
@@ -101,3 +104,35 @@ def test_hip_matches_golden(name):
         e = metrics.rel_err(got, W, S, 0.05)
         print(name, 'mixed vs golden', e.max())
         assert np.isfinite(got).all() and e.max() <= 1e-4, e.max()
+
+
+# mistakes planted in a copy of the oracle's C for mech19fx, each in a
+# branch only these vectors reach: (file, pattern, replacement)
+MUTATIONS = {
+    # SRI's d T^e factor dropped
+    'sri_dte_dropped': ('rxn_rates_pres_mod.c',
+                        r'\* \S+ \* pow\(T, \S+\) ', ''),
+    # a half-order concentration taken to the first power
+    'half_order_as_one': ('rxn_rates.c', r'pow\((C\[\d+\]), 0\.5\)',
+                          r'\1'),
+}
+
+
+@pytest.mark.parametrize('mutation', sorted(MUTATIONS))
+def test_golden_vectors_catch_planted_mistake(tmp_path, mutation):
+    name = 'mech19fx'
+    specs, reacs, fwd = mechanism(name)
+    src = tmp_path / 'src'
+    cfiles = emit_c.emit_c(specs, reacs, str(src), fwd)
+    fname, pat, repl = MUTATIONS[mutation]
+    path = src / fname
+    text, count = re.subn(pat, repl, path.read_text())
+    assert count > 0, mutation
+    path.write_text(text)
+    so = str(tmp_path / 'liboracle.so')
+    subprocess.run(['gcc'] + CFLAGS + ['-I', str(src), '-o', so] + cfiles +
+                   [os.path.join(ORACLE_DIR, 'driver.c'), '-lm'], check=True)
+    T, P, Y, W, S = _case(name)
+    got = Oracle(name, so).eval_stages(T, P, Y)['wdot']
+    e = metrics.rel_err(got, W, S, 1e-3)
+    assert e.max() > 1e-4, (mutation, e.max())
