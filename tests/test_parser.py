@@ -219,3 +219,65 @@ def test_species_names_ending_in_plus(tmp_path):
     terms, pdep, pdep_sp, thd = mi._split_side('CH2(S) + O2 (+AR)')
     assert [t[0].strip() for t in terms] == ['CH2(S)', 'O2']
     assert pdep and pdep_sp == 'AR' and not thd
+
+
+# ---- property-based: random reaction lines parse back to their terms ----
+
+from hypothesis import given, settings, strategies as hs  # noqa: E402
+
+_NAMES = ['H', 'O2', 'N2', 'HO2', 'OH', 'CH2(S)']
+_COEFS = [1, 2, 0.5, 1.5]
+
+
+def _side(draw):
+    names = draw(hs.lists(hs.sampled_from(_NAMES), min_size=1, max_size=3,
+                          unique=True))
+    return [(n, draw(hs.sampled_from(_COEFS))) for n in names]
+
+
+@hs.composite
+def _reaction(draw):
+    reac = _side(draw)
+    prod = _side(draw)
+    arrow = draw(hs.sampled_from(['=', '<=>', '=>']))
+    third = draw(hs.booleans())
+    A = draw(hs.sampled_from([1.0e5, 2.5e9, 3.1e12, 7.0e15]))
+    b = draw(hs.integers(-2000, 2000)) / 1000.0
+    E = draw(hs.integers(0, 50000)) * 1.0
+    return reac, prod, arrow, third, A, b, E
+
+
+def _text(side, third):
+    out = ' + '.join(('' if c == 1 else '{:g}'.format(c)) + n
+                     for n, c in side)
+    return out + (' + M' if third else '')
+
+
+@settings(max_examples=60, deadline=None)
+@given(_reaction())
+def test_random_reaction_lines_round_trip(tmp_path_factory, rx):
+    """Any reaction line over the thermo file's species parses to its own
+    species, coefficients, direction and third-body flag, with A converted
+    by 1000^(order - 1) (elementary) or 1000^order (third body, M not
+    counted) and E from cal/mole (mech_interpret.py:39-46, :441-455)."""
+    reac, prod, arrow, third, A, b, E = rx
+    line = '{} {} {}   {:.6E}  {}  {}'.format(
+        _text(reac, third), arrow, _text(prod, third), A, b, E)
+    d = tmp_path_factory.mktemp('rx')
+    m = d / 'mech.dat'
+    m.write_text('ELEMENTS\nH O N C\nEND\nSPECIES\n{}\nEND\nREACTIONS\n'
+                 '{}\nEND\n'.format(' '.join(_NAMES), line))
+    t = d / 'therm.dat'
+    t.write_text(THERMO)
+    _e, _s, reacs = mi.read_mech(str(m), str(t))
+    assert len(reacs) == 1, line
+    r = reacs[0]
+    assert list(zip(r.reac, r.reac_nu)) == reac, line
+    assert list(zip(r.prod, r.prod_nu)) == prod, line
+    assert r.rev == (arrow != '=>'), line
+    assert bool(r.thd_body) == third and not r.pdep, line
+    order = sum(c for _n, c in reac)
+    expect = A / 1000.0 ** (order if third else order - 1)
+    assert math.isclose(r.A, expect, rel_tol=1e-15), (line, r.A, expect)
+    assert r.b == b and math.isclose(r.E, E * 4.184 / 8.3144621,
+                                     rel_tol=1e-15), line
