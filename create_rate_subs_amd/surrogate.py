@@ -115,6 +115,33 @@ def _pad(n, m):
     return (n + m - 1) // m * m
 
 
+def gate_threshold(err, resid, target):
+    """(rmax, admitted fraction) of the residual gate for one error target.
+
+    States are ranked by their balance residual ``resid``; the admitted set
+    is the longest low-residual prefix whose worst error ``err`` stays
+    within ``target``, and rmax is the residual of its last state (a state
+    passes the gate when its residual is <= rmax, so a prefix never ends
+    inside a run of equal residuals).  Non-finite errors or residuals are
+    never admitted.  (0.0, 0.0) when no prefix qualifies.
+    """
+    e = torch.where(torch.isfinite(err), err, torch.full_like(err, 1e300))
+    r = torch.where(torch.isfinite(resid), resid,
+                    torch.full_like(resid, 1e300))
+    order = torch.argsort(r, stable=True)
+    rs = r[order]
+    worst = torch.cummax(e[order], dim=0).values
+    # a prefix may only end where the residual steps up: states tied with
+    # its last one would pass the gate too
+    ends = torch.ones_like(worst, dtype=torch.bool)
+    ends[:-1] = rs[1:] > rs[:-1]
+    ok = ((worst <= target) & ends & (rs < 1e300)).nonzero()
+    if ok.numel() == 0:
+        return 0.0, 0.0
+    last = int(ok.max().item())
+    return float(rs[last].item()), (last + 1) / err.numel()
+
+
 class Surrogate(object):
     """MLP [ln T, ln P, ln Y] -> arcsinh(wdot / s) for one mechanism."""
 
@@ -460,23 +487,12 @@ class Surrogate(object):
             e = metrics.normwise_err(W, W_exact)
         else:
             e = metrics.rel_err(W, W_exact, S, tau).max(dim=0).values
-        e = torch.where(torch.isfinite(e), e, torch.full_like(e, 1e300))
         r = self.residuals(W)
-        r = torch.where(torch.isfinite(r), r, torch.full_like(r, 1e300))
-        order = torch.argsort(r)
-        worst = torch.cummax(e[order], dim=0).values
-        n = T.numel()
-
-        def admit(tgt):
-            ok = (worst <= tgt).nonzero()
-            if ok.numel() == 0:
-                return 0.0, 0.0
-            last = int(ok.max().item())
-            return float(r[order[last]].item()), (last + 1) / n
-
-        table = [dict(target=t, rmax=admit(t)[0], admitted=admit(t)[1])
-                 for t in curve]
-        rmax, frac = admit(target)
+        table = []
+        for t in curve:
+            rm, fr = gate_threshold(e, r, t)
+            table.append(dict(target=t, rmax=rm, admitted=fr))
+        rmax, frac = gate_threshold(e, r, target)
         self.gate.rmax = rmax
         return rmax, frac, table
 
