@@ -160,3 +160,16 @@ def test_two_wave_choice(nsp, two):
     # so is a layout the two-wave rows do not fit beside
     for opts in ({'nasa': 'lds'}, {'g_store': 'lds'}, {'kc': 'species'}):
         assert hip_emitter.resolve_opts(nsp, opts)['waves_per_simd'] is None
+
+
+def test_wdot_block_option():
+    """The wdot kernels' block size is an emitter option: 128 threads give
+    two waves' rows per block and a matching launch; sizes that are not a
+    multiple of the wave are refused."""
+    ir = _ir('mech13')
+    text = hip_emitter.emit_hip_source(ir, {'block': 128})
+    assert 'constexpr int kWdotBlock = 128;' in text
+    assert 'sGl[kWdotBlock / 64]' in text
+    assert 'k_wdot_f64<<<crs::grid_for(n, kWdotBlock), kWdotBlock' in text
+    with pytest.raises(ValueError):
+        hip_emitter.emit_hip_source(ir, {'block': 100})
