@@ -1,9 +1,10 @@
 """Write tests/golden/wdot_kat.json: known-answer wdot vectors.
 
-For four synthetic mechanisms that between them hold every rate form of
+For five synthetic mechanisms that between them hold every rate form of
 the path (mech13: elementary, three-body, Troe, Lindemann; mech13co: the
 same with a reactive species last, so the implied species' terms are
-non-zero; mech21pc: PLOG and Chebyshev beside them; mech19fx: 3-parameter
+non-zero; mech17tm: species with two different T_mid, 1000 K and 1300 K;
+mech21pc: PLOG and Chebyshev beside them; mech19fx: 3-parameter
 Troe, 5-parameter SRI, a named collider, chemically activated reactions,
 split REV reactions and non-integer orders), eight seeded states each,
 wdot and the constant-pressure dydt evaluated in mpmath at 50 digits from
@@ -34,17 +35,21 @@ from create_rate_subs_amd import states, workloads  # noqa: E402
 from create_rate_subs_amd.create_rate_subs import load_mechanism  # noqa: E402
 from oracle import numpy_ref  # noqa: E402
 
-MECHS = ('mech13', 'mech13co', 'mech21pc', 'mech19fx')
+MECHS = ('mech13', 'mech13co', 'mech17tm', 'mech21pc', 'mech19fx')
 COUNT = 8
 SEED = 41
 OUT = os.path.join(HERE, 'wdot_kat.json')
+# mech17tm's states straddle its two T_mid (1000 K, 1300 K), where the
+# species disagree about which NASA range applies
+T_RANGE = {'mech17tm': (950.0, 1350.0)}
 
 
 def make(name):
     mpath, tpath = workloads.mechanism_files(name)
     _e, specs, reacs, _f = load_mechanism(
         mpath, tpath, last_spec=workloads.last_species(name))
-    T, P, Y = states.random_states(len(specs), COUNT, SEED)
+    T, P, Y = states.random_states(len(specs), COUNT, SEED,
+                                   T_RANGE.get(name, (800.0, 2500.0)))
     ref = numpy_ref.evaluate(specs, reacs, T, P, Y)
     S = numpy_ref.gross_scale(specs, reacs, ref['fwd'], ref['rev'],
                               ref['pm'])
