@@ -14,7 +14,6 @@ C oracle's throughput on the host cores over a bounded sample.
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -61,28 +60,31 @@ def log(msg):
     print('[bench] ' + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(name, T, P, Y, sample, threads):
-    """C oracle (restated generated C, gcc -O3 -fopenmp, FP64) on host cores."""
-    from oracle.build import Oracle, build, lib_path
-    if not os.path.exists(lib_path(name)):
-        build(name)
-    o = Oracle(name)
-    m = min(sample, T.size)
-    Ts, Ps = np.ascontiguousarray(T[:m]), np.ascontiguousarray(P[:m])
-    Ys = np.ascontiguousarray(Y[:, :m])
-    o.eval_wdot(Ts, Ps, Ys, threads)          # warm-up
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        o.eval_wdot(Ts, Ps, Ys, threads)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return dict(value=m / t, unit='states/s', cores=threads, kind='port',
-                sample='{} of the workload\'s {} states, {} OpenMP threads, '
-                       'median of 3 after 1 warm-up ({:.3f} s each, {:.1f} s '
-                       'of CPU work timed); host {}'
-                       .format(m, T.size, threads, t, 3 * t * threads,
-                               platform.processor() or platform.machine()))
+def cpu_threads(requested=0):
+    """Threads for the CPU baseline: the CPUs this process may use, capped by
+    OMP_NUM_THREADS when set (a GPU box grants each GPU's job its share of
+    the host, 16 CPUs, and sets OMP_NUM_THREADS to it)."""
+    if requested:
+        return requested
+    allowed = len(os.sched_getaffinity(0))
+    env = os.environ.get('OMP_NUM_THREADS')
+    return min(allowed, int(env)) if env else allowed
+
+
+def cpu_baseline(workload, sample, threads):
+    """The C oracle (restated generated C, gcc -O3 -fopenmp, FP64) on the
+    host cores, in a child process (oracle/cpu_baseline.py) so that
+    OMP_PROC_BIND=close and OMP_PLACES=cores reach libgomp at its start."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND='close', OMP_PLACES='cores',
+               OMP_NUM_THREADS=str(threads))
+    res = subprocess.run([sys.executable, '-m', 'oracle.cpu_baseline',
+                          workload, str(sample), str(threads)], cwd=ROOT,
+                         env=env, capture_output=True, text=True,
+                         timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError('CPU baseline failed:\n' + res.stderr[-2000:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
 
 
 def measured_traffic(kernel, workload, n):
@@ -385,7 +387,8 @@ def main():
     by = cost.bytes_per_state(nsp)
     achieved = fl * n / (kern_ms * 1e-3) / 1e12
     traffic, traffic_src = measured_traffic('k_wdot_f64', wl.name, n)
-    roofline = dict(bound='mfma', achieved=achieved, peak=FP64_PEAK_TFLOPS,
+    roofline = dict(bound='fp64_valu', achieved=achieved,
+                    peak=FP64_PEAK_TFLOPS,
                     unit='TFLOP/s', frac=achieved / FP64_PEAK_TFLOPS,
                     traffic=traffic, traffic_source=traffic_src,
                     kernel='k_wdot_f64',
@@ -408,10 +411,8 @@ def main():
                             mechanism=wl.mech, states=n, mode='fp64'),
                 roofline=roofline)
     if not args.no_cpu:
-        threads = args.cpu_threads or int(os.environ.get(
-            'OMP_NUM_THREADS', min(16, os.cpu_count() or 1)))
-        line['cpu_baseline'] = cpu_baseline(wl.mech, T, P, Y,
-                                            args.cpu_sample, threads)
+        line['cpu_baseline'] = cpu_baseline(wl.name, args.cpu_sample,
+                                            cpu_threads(args.cpu_threads))
         log('cpu baseline: {:.3e} states/s'.format(
             line['cpu_baseline']['value']))
     line['dydt'] = dydt_section(args, torch, lib, Td, Pd, Yd)

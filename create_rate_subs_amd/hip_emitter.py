@@ -715,6 +715,23 @@ DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
 # per-lane G rows per wave at two waves per SIMD (two 4-wave blocks per CU)
 _ROWS_2W = (80 * 1024 - 512) // 2048
 
+_LDS_PER_CU = 160 * 1024
+
+
+def _lane_rows(block, waves_per_simd, table_bytes):
+    """Per-lane FP64 LDS rows per wave that let the wdot kernels reach
+    ``waves_per_simd`` waves per SIMD with ``block``-thread blocks.
+
+    Every wave of a block owns rows x 512 bytes; a block also holds
+    ``table_bytes`` of shared tables and 512 bytes of slack.  The CU's
+    160 KB is split over the blocks that must be resident together: enough
+    for 4 x waves_per_simd waves, and at least one block."""
+    wpb = block // 64
+    waves_cu = max(4 * waves_per_simd, wpb)
+    blocks_cu = max(1, waves_cu // wpb)
+    per_block = _LDS_PER_CU // blocks_cu - table_bytes - 512
+    return max(0, min(78, per_block // (wpb * 512)))
+
 
 def resolve_opts(nsp, opts=None):
     """Emitter options with 'auto' choices made for an NSP-species mechanism.
@@ -724,22 +741,42 @@ def resolve_opts(nsp, opts=None):
     nasa='lds') takes 128 B per species per block.  nasa='auto' takes the
     table where the H/G form fits beside it.  waves_per_simd='auto' gives
     40-56 species two waves per SIMD instead: 39 rows per wave, the other
-    G_k in registers, no table.
+    G_k in registers, no table.  Row counts follow the wdot block size
+    (option ``block``, 256 threads by default; _lane_rows).
+
+    The block-wide LDS layouts of earlier measurements (g_store='lds' or
+    'lds_sync', c_store='lds': one [rows][block] array shared by the block)
+    are refused: they need a barrier after the per-state early exits of
+    partial blocks and of the subset kernel, and they lost to the per-lane
+    rows.
     """
     o = dict(DEFAULT_OPTS, **(opts or {}))
+    if o.get('contraction', 'sparse') != 'sparse':
+        raise ValueError(
+            "contraction={!r}: only the sparse per-lane accumulation is "
+            "built.  The FP32-accumulate MFMA GEMM over nu^T was measured "
+            "2.8x slower and overflows FP32 (reverse rates reach 4e56), so it "
+            "was removed".format(o.get('contraction')))
+    if o.get('g_store') in ('lds', 'lds_sync') or o.get('c_store') == 'lds':
+        raise ValueError("block-wide LDS layouts (g_store='lds'/'lds_sync', "
+                         "c_store='lds') are not supported; use 'lane_lds'")
+    block = int(o.get('block') or 256)
+    if block % 64 or not 64 <= block <= 1024:
+        raise ValueError('block must be a multiple of 64 in [64, 1024]')
 
-    def rows(table):
-        return min(78, (160 * 1024 - table - 1024) // 2048)
+    def rows(table, wps=1):
+        return _lane_rows(block, wps, table)
     if o.get('waves_per_simd') == 'auto':
         # only the per-lane G-row layout, and no NASA table (it would not
         # leave room for a second block)
-        two = (_ROWS_2W < nsp <= 56 and 'lds_species' not in o and
+        rows2 = rows(0, 2)
+        two = (rows2 < nsp <= 56 and 'lds_species' not in o and
                o.get('kc') in ('auto', 'species_hg') and
                o.get('g_store', 'lane_lds') == 'lane_lds' and
                o.get('nasa') != 'lds')
         o['waves_per_simd'] = 2 if two else None
         if two:
-            o['lds_species'] = _ROWS_2W
+            o['lds_species'] = rows2
             o['kc'] = 'species_hg'
             o.setdefault('g_store', 'lane_lds')
             o.setdefault('g_pick', 'retire')
@@ -747,11 +784,14 @@ def resolve_opts(nsp, opts=None):
             o.setdefault('lds_prefetch', 1)
             if o.get('nasa') == 'auto':
                 o['nasa'] = None
+    wps = int(o.get('waves_per_simd') or 1)
     if o.get('nasa') == 'auto':
-        fits = nsp <= rows(128 * nsp) and o.get('kc') in ('auto', 'species_hg')
+        fits = nsp <= rows(128 * nsp, wps) and o.get('kc') in (
+            'auto', 'species_hg')
         o['nasa'] = 'lds' if fits else None
     if 'lds_species' not in o:
-        o['lds_species'] = rows(128 * nsp if o.get('nasa') == 'lds' else 0)
+        o['lds_species'] = rows(128 * nsp if o.get('nasa') == 'lds' else 0,
+                                wps)
     if o.get('kc') == 'auto':
         if nsp <= o['lds_species']:
             o['kc'] = 'species_hg'
@@ -1064,7 +1104,7 @@ def _nasa_table_values(ir):
     return vals
 
 
-def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
+def _species_gibbs_lines(ir, K, tmids, store_inverse=False,
                          every=False, expfn='exp', nasa_lds=False):
     """Per-state G_k = exp(S_k/R - H_k/RT) and its inverse, once per species.
 
@@ -1097,15 +1137,6 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
         expr = ' * '.join([base] * abs(d))
         out.append('const double cpow{}{} = {};'.format(
             'p' if d > 0 else 'm', abs(d), expr))
-    if lds:
-        # one row per species, one column per lane: conflict-free access
-        # volatile keeps every access a real LDS access: the compiler would
-        # otherwise forward the stores into registers and drop the array
-        out.append('__shared__ double sG_[{}][crs::kBlock];'.format(
-            len(_gibbs_species(ir))))
-        out.append('volatile double (*sG)[crs::kBlock] = sG_;')
-        out.append('const int lx = threadIdx.x;')
-    slots = {k: s for s, k in enumerate(_gibbs_species(ir))}
     order = sorted(need_g | need_gi)
 
     def nasa_reads(k):
@@ -1141,14 +1172,9 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False, lds=False,
                         terms.append('{} * p{}{}_{}'.format(K(cv), half, gi,
                                                              bi))
             out.append('const double smh{} = {};'.format(k, ' + '.join(terms)))
-        if lds:
-            out.append('sG[{s}][lx] = exp(smh{k});'.format(s=slots[k], k=k))
-            continue
         out.append('const double G{k} = {e}(smh{k});'.format(k=k, e=expfn))
         if k in need_gi:
             out.append('const double Gi{k} = 1.0 / G{k};'.format(k=k))
-    if lds == 'sync':
-        out.append('__syncthreads();')
     return out
 
 
@@ -1297,26 +1323,6 @@ def _mixed_pm_lines(x, K, kf_x):
 
 
 _C_REF = re.compile(r'\bC\[(\d+)\]')
-
-
-def _concentrations_to_lds(lines, nsp):
-    """Keep the per-state concentrations in LDS instead of registers.
-
-    One [NSP][block] FP64 array, one column per lane (conflict-free), read
-    through a volatile pointer so every use is an LDS load rather than a
-    register the compiler keeps live across the whole reaction list.
-    """
-    out = []
-    for l in lines:
-        if l == 'double C[{}];'.format(nsp):
-            out.append('__shared__ double sC[{}][crs::kBlock];'.format(nsp))
-            out.append('const int lxc = threadIdx.x;')
-            continue
-        if l == _CONC_DONE:
-            # concentrations complete: publish them, then read back
-            out.append('__syncthreads();')
-        out.append(_C_REF.sub(r'sC[\1][lxc]', l))
-    return out
 
 
 def _concentrations_to_lane_lds(lines, nsp, nl, block='crs::kBlock'):
@@ -1505,73 +1511,6 @@ def _species_updates(x, seen):
     return out
 
 
-_MF_CHUNK = 32   # reactions per contraction step (8 MFMA K-steps of 4)
-
-
-def _mfma_nu_table(ir, rxns):
-    """nu^T in v_mfma_f32_16x16x4f32 A-operand order.
-
-    For chunk c of _MF_CHUNK reactions (in emission order), species tile st
-    (16 species), K-step ks (4 reactions) and lane l, the value is
-    nu[16 st + l % 16][reaction 32 c + 4 ks + l / 16] (0 outside).
-    """
-    nst = (ir.nsp + 15) // 16
-    nch = (len(rxns) + _MF_CHUNK - 1) // _MF_CHUNK
-    vals = []
-    for c in range(nch):
-        for st in range(nst):
-            for ks in range(_MF_CHUNK // 4):
-                for lane in range(64):
-                    k = 16 * st + lane % 16
-                    j = _MF_CHUNK * c + 4 * ks + lane // 16
-                    v = 0.0
-                    if k < ir.nsp and j < len(rxns):
-                        for kk, nu in rxns[j].spec:
-                            if kk == k:
-                                v = float(nu)
-                    vals.append(v)
-    return nst, nch, vals
-
-
-def _mfma_chunk_end(pos, nst):
-    """Contraction after reaction position ``pos`` closes a chunk (the
-    rest of a partial last chunk is zero-filled first)."""
-    c = pos // _MF_CHUNK
-    out = []
-    for slot in range(pos % _MF_CHUNK + 1, _MF_CHUNK):
-        out.append('sQ[threadIdx.x >> 6][{}][threadIdx.x & 63] = 0.0f;'.format(
-            slot))
-    out.append('__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
-    return out + _mfma_contract_lines(c, nst)
-
-
-def _mfma_contract_lines(c, nst):
-    """D[st][ct] += nu^T(chunk c) x Q(chunk) for the wave's 4 state tiles."""
-    out = ['{  // contraction of chunk %d on MFMA' % c,
-           '    const float *qw = &sQ[threadIdx.x >> 6][0][0];',
-           '    const int lq = threadIdx.x & 63;']
-    out.append('    #pragma unroll')
-    out.append('    for (int ks = 0; ks < {}; ++ks) {{'.format(_MF_CHUNK // 4))
-    out.append('        float bq[4];')
-    out.append('        #pragma unroll')
-    out.append('        for (int ct = 0; ct < 4; ++ct)')
-    out.append('            bq[ct] = qw[(4 * ks + (lq >> 4)) * 64 + 16 * ct + '
-               '(lq & 15)];')
-    out.append('        #pragma unroll')
-    out.append('        for (int st = 0; st < {}; ++st) {{'.format(nst))
-    out.append('            const float a = crs_nuA[(({c} * {nst} + st) * {ks} + ks)'
-               ' * 64 + lq];'.format(c=c, nst=nst, ks=_MF_CHUNK // 4))
-    out.append('            #pragma unroll')
-    out.append('            for (int ct = 0; ct < 4; ++ct)')
-    out.append('                Dq[st][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32('
-               'a, bq[ct], Dq[st][ct], 0, 0, 0);')
-    out.append('        }')
-    out.append('    }')
-    out.append('    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
-    out.append('}')
-    return out
-
-
 def _nasa_masked(ir, K, k, tmids, coef_of, powers):
     """sum_b c_b x_b over the masked basis of species k's T_mid group;
     coef_of(a) gives the coefficients for NASA set a, powers the basis
@@ -1718,12 +1657,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     if species_kc:
         lines += _species_gibbs_lines(ir, K, tmids,
                                       opts.get('kc_inverse') == 'store',
-                                      {'lds': True, 'lds_sync': 'sync'}.get(
-                                          opts.get('g_store'), False),
                                       every=hg, expfn=fexp, nasa_lds=nasa_lds)
-    gslots = {k: s for s, k in enumerate(_gibbs_species(ir))}
-    gname = ((lambda k: 'sG[{}][lx]'.format(gslots[k]))
-             if opts.get('g_store') in ('lds', 'lds_sync') else None)
     g_lane = hg and opts.get('g_store') == 'lane_lds'
     # LDS budget: one wave per SIMD leaves 40 KB per wave, 80 FP64 rows
     n_gl = min(nsp, opts.get('lds_species', 78)) if g_lane else 0
@@ -1763,27 +1697,14 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     rx_start = len(lines)
     last = _last_touch(ir, rxns)
     seen = [False] * nsp
-    mfma = mixed and opts.get('contraction') == 'mfma'
-    if mfma:
-        nst = (nsp + 15) // 16
-        lines.append('__shared__ float sQ[{} / 64][{}][64];'.format(
-            opts.get('block_expr', 'crs::kBlock'),
-            _MF_CHUNK))
-        lines.append('floatx4 Dq[{}][4];'.format(nst))
-        lines.append('#pragma unroll')
-        lines.append('for (int a = 0; a < {}; ++a)'.format(nst))
-        lines.append('    for (int b = 0; b < 4; ++b)')
-        lines.append('        Dq[a][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};')
-        last = {}
-    else:
-        lines.append('double w[{}];'.format(nsp))
-        for k in range(nsp):
-            if k not in last:
-                if dydt:
-                    if k < nsp - 1:
-                        lines.append('W[{r} * ld + i] = 0.0;'.format(r=k + 1))
-                else:
-                    lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
+    lines.append('double w[{}];'.format(nsp))
+    for k in range(nsp):
+        if k not in last:
+            if dydt:
+                if k < nsp - 1:
+                    lines.append('W[{r} * ld + i] = 0.0;'.format(r=k + 1))
+            else:
+                lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
     for pos, x in enumerate(rxns):
         if x.plog:
             inner, xf = _plog_fast_lines(x, K, expfn), None
@@ -1800,17 +1721,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                           _fast_pm_lines(x, K, xf, arr2))
                 pm = ' * pm'
             inner += _hg_net_rate(x, gref, pm)
-            if mfma:
-                inner.append('sQ[threadIdx.x >> 6][{}][threadIdx.x & 63] = '
-                             '(float)q;'.format(pos % _MF_CHUNK))
-            else:
-                inner += _species_updates(x, seen)
+            inner += _species_updates(x, seen)
             lines.append('{  // rxn %d' % x.index)
             lines += ['    ' + l for l in inner]
             lines.append('}')
-            if mfma and (pos % _MF_CHUNK == _MF_CHUNK - 1 or
-                         pos == len(rxns) - 1):
-                lines += _mfma_chunk_end(pos, nst)
             for k, _nu in x.spec:
                 if last.get(k) == x.index:
                     lines += (_dydt_finish_lines(ir, K, tmids, k,
@@ -1830,7 +1744,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                 # concentration product cannot underflow prematurely
                 inner.append('const double r = {}(kf * {});'.format(
                     _conc_prod(x.prod), _species_kc_inverse(
-                        x, opts.get('kc_inverse') == 'store', gname)))
+                        x, opts.get('kc_inverse') == 'store')))
             else:
                 # remap this reaction's Tmid groups onto the global masks
                 groups = []
@@ -1868,16 +1782,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                       _fast_pm_lines(x, K, xf, arr2))
             q = q + ' * pm'
         inner.append('const double q = {};'.format(q))
-        if mfma:
-            inner.append('sQ[threadIdx.x >> 6][{}][threadIdx.x & 63] = '
-                         '(float)q;'.format(pos % _MF_CHUNK))
-        else:
-            inner += _species_updates(x, seen)
+        inner += _species_updates(x, seen)
         lines.append('{  // rxn %d' % x.index)
         lines += ['    ' + l for l in inner]
         lines.append('}')
-        if mfma and (pos % _MF_CHUNK == _MF_CHUNK - 1 or pos == len(rxns) - 1):
-            lines += _mfma_chunk_end(pos, nst)
         for k, _nu in x.spec:
             if last.get(k) == x.index:
                 lines += (_dydt_finish_lines(ir, K, tmids, k,
@@ -1889,26 +1797,6 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     if dydt:
         # energy equation, rate_subs.py:1910-1925
         lines.append('W[i] = (-1.0 / (rho * cp_avg)) * esum;')
-    if mfma:
-        # D[st][ct]: lane l holds species 16 st + 4 (l / 16) + t of the
-        # state in lane 16 ct + l % 16 of this wave
-        lines.append('{')
-        lines.append('    const int lq = threadIdx.x & 63;')
-        lines.append('    #pragma unroll')
-        lines.append('    for (int ct = 0; ct < 4; ++ct) {')
-        lines.append('        const int src = 16 * ct + (lq & 15);')
-        lines.append('        const long long iq = __shfl((long long)i, src);')
-        lines.append('        const int ok = __shfl((int)live, src);')
-        lines.append('        #pragma unroll')
-        lines.append('        for (int st = 0; st < {}; ++st)'.format(nst))
-        lines.append('            #pragma unroll')
-        lines.append('            for (int t = 0; t < 4; ++t) {')
-        lines.append('                const int k = 16 * st + 4 * (lq >> 4) + t;')
-        lines.append('                if (ok && k < {})'.format(nsp))
-        lines.append('                    W[k * ld + iq] = (double)Dq[st][ct][t];')
-        lines.append('            }')
-        lines.append('    }')
-        lines.append('}')
     if hg:
         # concentrations after the H_k lines: C_k = G_k * H_k
         def c_of(mo):
@@ -1925,9 +1813,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
             rx = out
         lines = lines[:rx_start] + rx
     sources = [('gb', 'gl', True)] if hg and g_lane else []
-    if opts.get('c_store') == 'lds':
-        lines = _concentrations_to_lds(lines, ir.nsp)
-    elif opts.get('c_store') == 'lane_lds':
+    if opts.get('c_store') == 'lane_lds':
         lines = _concentrations_to_lane_lds(
             lines, ir.nsp, min(ir.nsp, opts.get('lds_species', 78)),
             opts.get('block_expr', 'crs::kBlock'))
@@ -2167,7 +2053,7 @@ def emit_hip_source(ir, opts=None):
 
     # ---- constant-pressure dydt (FP64), the caller of the path -------
     dydt_lines, dydt_fill = fused_state(
-        'dydt', dict(dopts, contraction='sparse', preload=preload), dydt=True)
+        'dydt', dict(dopts, preload=preload), dydt=True)
     body.append(
         '__device__ __forceinline__ void dydt_f64_state(\n'
         '    const double *__restrict__ K, const double *__restrict__ P_,\n'
@@ -2204,11 +2090,15 @@ def emit_hip_source(ir, opts=None):
         '//   2: acc += 2 k,   ytmp = y + h/2 k\n'
         '//   3: acc += 2 k,   ytmp = y + h k\n'
         '//   4: y += h/6 (acc + k); a state whose T first reaches\n'
-        '//      T0 + dT_ign gets t_ign = t_new (when t_ign is given)\n'
+        '//      T0 + dT_ign gets t_ign = t_step, the time at the start of\n'
+        '//      this step, as the reference records t before its t += h\n'
+        '//      (create_rate_subs.py:267-276).  NaN marks a state that has\n'
+        '//      not ignited yet, so any finite time (a negative t0 too) is\n'
+        '//      kept once set.\n'
         '__global__ __launch_bounds__(crs::kBlock) void k_rk4_combine(\n'
         '    double *__restrict__ y, double *__restrict__ ytmp,\n'
         '    double *__restrict__ acc, const double *__restrict__ k,\n'
-        '    int64_t n, int64_t ld, int stage, double h, double t_new,\n'
+        '    int64_t n, int64_t ld, int stage, double h, double t_step,\n'
         '    const double *__restrict__ T0, double *__restrict__ t_ign,\n'
         '    double dT_ign)\n{\n'
         '#pragma clang fp contract(off)\n'
@@ -2230,48 +2120,15 @@ def emit_hip_source(ir, opts=None):
         '            y[o] += (h / 6.0) * (acc[o] + kv);\n'
         '        }\n'
         '    }\n'
-        '    if (stage == 4 && t_ign != nullptr && !(t_ign[i] >= 0.0) &&\n'
+        '    if (stage == 4 && t_ign != nullptr && t_ign[i] != t_ign[i] &&\n'
         '        y[i] >= T0[i] + dT_ign)\n'
-        '        t_ign[i] = t_new;\n}\n\n')
+        '        t_ign[i] = t_step;\n}\n\n')
 
     # ---- fused mixed-precision per-state body -------------------------
-    mfma_mixed = opts.get('contraction') == 'mfma'
     if opts.get('mixed', True):
         mixed_lines, mixed_fill = fused_state(
-            'mixed', dict(opts, preload=preload and not mfma_mixed,
-                          block_expr='kWdotBlock'),
+            'mixed', dict(opts, preload=preload, block_expr='kWdotBlock'),
             mixed=True)
-    if opts.get('mixed', True) and opts.get('contraction') == 'mfma':
-        rx = _retiring_order(ir) if opts.get('order') == 'retire' else ir.rxns
-        nst, nch, vals = _mfma_nu_table(ir, rx)
-        body.append('typedef __attribute__((__vector_size__(4 * sizeof(float)))) '
-                    'float floatx4;\n')
-        body.append('__device__ const float crs_nuA[{}] = {{\n'.format(len(vals)) +
-                    ''.join('{}f,{}'.format(repr(v), '\n' if q % 16 == 15 else ' ')
-                            for q, v in enumerate(vals)) + '};\n\n')
-        body.append(
-            '__device__ __forceinline__ void wdot_mixed_state(\n'
-            '    const double *__restrict__ K,\n'
-            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
-            '    const double *__restrict__ Y, double *__restrict__ W,\n'
-            '    int64_t ld, int64_t i, bool live)\n{\n' +
-            _indent(mixed_lines, 4) + '}\n\n')
-        body.append(
-            '__global__ __launch_bounds__(kWdotBlock' + lb + ') void k_wdot_mixed(\n'
-            '    const double *__restrict__ K,\n'
-            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
-            '    const double *__restrict__ Y, double *__restrict__ W,\n'
-            '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n'
-            + mixed_fill +
-            '    // whole waves beyond m leave; within a wave every lane runs (the\n'
-            '    // contraction exchanges data across lanes), dead lanes on state 0\n'
-            '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
-            '    if ((t & ~(int64_t)63) >= m) return;\n'
-            '    int64_t i = (t < m) ? (idx ? (int64_t)idx[t] : t) : 0;\n'
-            '    const bool live = t < m && i >= 0 && i < ld;\n'
-            '    if (!live) i = 0;\n'
-            '    wdot_mixed_state(K, T_, P_, Y, W, ld, i, live);\n}\n\n')
-    elif opts.get('mixed', True):
         body.append(
             '__device__ __forceinline__ void wdot_mixed_state(\n'
             '    const double *__restrict__ K,\n'
@@ -2519,7 +2376,7 @@ int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
             k_dydt_f64<<<grid, crs::kBlock, 0, s>>>(K, P, stage == 1 ? y : ytmp,
                                                     k, n, ld);
             k_rk4_combine<<<grid, crs::kBlock, 0, s>>>(
-                y, ytmp, acc, k, n, ld, stage, h, t + h, T0, t_ign, dT_ign);
+                y, ytmp, acc, k, n, ld, stage, h, t, T0, t_ign, dT_ign);
         }}
         t += h;
     }}

@@ -36,15 +36,17 @@ def integrate(lib, P, y, t0=0.0, tend=1.0e-7, h=1.0e-8, graph=True,
               dT_ign=400.0):
     """Advance y [NSP or NSP+1][n] (device, FP64, row 0 = T) in place from t0
     to tend in steps of h.  Returns a dict with the step count, the
-    per-state ignition time (-1 where T never reached T0 + dT_ign) and the
-    measured time per step."""
+    per-state ignition time (the start time of the first step that ends
+    with T >= T0 + dT_ign, as the reference records it; NaN where T never
+    got there) and the measured time per step."""
     n = P.numel()
     steps = loop_steps(t0, tend, h)
     ld = y.stride(0)
     work = torch.empty((3 * lib.nsp, ld), dtype=torch.float64,
                        device=y.device)[:, :n]
     T0 = y[0].clone()
-    t_ign = torch.full((n,), -1.0, dtype=torch.float64, device=y.device)
+    t_ign = torch.full((n,), float('nan'), dtype=torch.float64,
+                       device=y.device)
     # one eager call on a single state sets the library up before capture
     lib.dydt(P[:1].contiguous(), y[:, :1], out=work[:y.shape[0], :1])
     torch.cuda.synchronize()

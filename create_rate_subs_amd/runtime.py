@@ -199,7 +199,9 @@ class MechLib(object):
     def rk4(self, P, y, t0, h, nsteps, work=None, T0=None, t_ign=None,
             dT_ign=400.0):
         """nsteps RK4 steps of the CONP system on y (updated in place);
-        returns y.  ``work``: [3 NSP][ld] scratch (allocated if None)."""
+        returns y.  ``work``: [3 NSP][ld] scratch (allocated if None).
+        With T0 and t_ign: t_ign (initialised to NaN) gets the start time
+        of the first step after which T >= T0 + dT_ign."""
         n = P.numel()
         self._vec(P, n, 'P')
         rows = y.shape[0] if y.dim() == 2 else -1

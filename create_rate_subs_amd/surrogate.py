@@ -20,6 +20,7 @@ This is torch-free except for device buffers; every kernel is ours.
 """
 
 import ctypes
+import hashlib
 import json
 import math
 import os
@@ -29,7 +30,7 @@ import torch
 
 from . import metrics
 
-__all__ = ['MlpLib', 'Surrogate', 'load_mlp']
+__all__ = ['MlpLib', 'Surrogate', 'load_mlp', 'mechanism_hash']
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MLP_LIB = os.path.join(HERE, '_build', 'libcrs_mlp.so')
@@ -140,6 +141,37 @@ def gate_threshold(err, resid, target):
         return 0.0, 0.0
     last = int(ok.max().item())
     return float(rs[last].item()), (last + 1) / err.numel()
+
+
+def mechanism_hash(specs, reacs):
+    """SHA-256 of a mechanism's tables, in the library's species order.
+
+    Covers what the rates depend on: species names, molecular weights,
+    NASA-7 ranges and coefficients, and every reaction field of ReacInfo
+    (stoichiometry, Arrhenius, reversibility, third-body efficiencies,
+    falloff, Troe/SRI, PLOG, Chebyshev).  Floats enter by their exact
+    IEEE-754 bits (float.hex), so two mechanisms differing in any constant
+    hash differently.
+    """
+    def canon(v):
+        if isinstance(v, np.ndarray):
+            return [canon(x) for x in v.tolist()]
+        if isinstance(v, (list, tuple)):
+            return [canon(x) for x in v]
+        if isinstance(v, (bool, np.bool_)):
+            return bool(v)
+        if isinstance(v, (int, np.integer)):
+            return int(v)
+        if isinstance(v, (float, np.floating)):
+            return float(v).hex()
+        return str(v)
+    doc = dict(
+        species=[[sp.name, canon(sp.mw), canon(sp.Trange), canon(sp.lo),
+                  canon(sp.hi)] for sp in specs],
+        reactions=[{k: canon(v) for k, v in sorted(vars(r).items())}
+                   for r in reacs])
+    text = json.dumps(doc, sort_keys=True, separators=(',', ':'))
+    return hashlib.sha256(text.encode()).hexdigest()
 
 
 class Surrogate(object):
@@ -497,37 +529,70 @@ class Surrogate(object):
         return rmax, frac, table
 
     # -- checkpoint ---------------------------------------------------------
-    def save(self, directory):
-        os.makedirs(directory, exist_ok=True)
+    def _array_specs(self):
+        """Arrays a checkpoint holds: name -> (tensor, dtype, shape)."""
         arrays = dict(p=self.p, m1=self.m1, m2=self.m2, mu=self.mu,
                       sd=self.sd, scale=self.scale, ymu=self.ymu,
                       ysd=self.ysd)
+        return {k: (t, str(t.dtype).replace('torch.', ''), list(t.shape))
+                for k, t in arrays.items()}
+
+    def save(self, directory):
+        os.makedirs(directory, exist_ok=True)
         manifest = dict(nsp=self.nsp, hidden=self.hid,
                         shapes=self.shapes, step=self.step,
+                        mechanism=mechanism_hash(self.specs, self.reacs),
                         gate=dict(t_lo=self.gate.t_lo, t_hi=self.gate.t_hi,
                                   p_lo=self.gate.p_lo, p_hi=self.gate.p_hi,
                                   rmax=self.gate.rmax), files={})
-        for k, t in arrays.items():
+        for k, (t, dtype, shape) in self._array_specs().items():
             a = t.detach().cpu().numpy()
             fn = k + '.bin'
             a.astype(a.dtype.newbyteorder('<')).tofile(
                 os.path.join(directory, fn))
-            manifest['files'][k] = dict(file=fn, dtype=str(a.dtype),
-                                        shape=list(a.shape))
+            manifest['files'][k] = dict(file=fn, dtype=dtype, shape=shape)
         with open(os.path.join(directory, 'manifest.json'), 'w') as fh:
             json.dump(manifest, fh, indent=1)
 
     def load(self, directory):
+        """Restore a checkpoint written by save() for this mechanism.
+
+        Refuses (ValueError) a checkpoint of another mechanism (hash of the
+        species, thermo and reaction tables), of another network shape, or
+        whose manifest names arrays other than the expected ones or with
+        another dtype or shape; nothing is modified then.
+        """
         with open(os.path.join(directory, 'manifest.json')) as fh:
             man = json.load(fh)
+        if man.get('mechanism') != mechanism_hash(self.specs, self.reacs):
+            raise ValueError('checkpoint was trained on another mechanism')
         if man['nsp'] != self.nsp or [tuple(s) for s in man['shapes']] != \
                 [tuple(s) for s in self.shapes]:
             raise ValueError('checkpoint does not match this network')
-        for k, spec in man['files'].items():
+        expect = self._array_specs()
+        files = man.get('files', {})
+        if set(files) != set(expect):
+            raise ValueError('checkpoint arrays {} differ from {}'.format(
+                sorted(files), sorted(expect)))
+        loaded = {}
+        for k, (t, dtype, shape) in expect.items():
+            spec = files[k]
+            if spec.get('dtype') != dtype or list(spec.get('shape', ())) != \
+                    shape or os.path.basename(spec.get('file', '')) != \
+                    spec.get('file'):
+                raise ValueError('checkpoint array {} has {} {} (expected '
+                                 '{} {})'.format(k, spec.get('dtype'),
+                                                 spec.get('shape'), dtype,
+                                                 shape))
             a = np.fromfile(os.path.join(directory, spec['file']),
-                            dtype=np.dtype(spec['dtype']).newbyteorder('<'))
-            getattr(self, k).copy_(torch.from_numpy(
-                a.reshape(spec['shape']).astype(spec['dtype'])))
+                            dtype=np.dtype(dtype).newbyteorder('<'))
+            if a.size != int(np.prod(shape)):
+                raise ValueError('checkpoint array {} holds {} values, '
+                                 'expected {}'.format(k, a.size,
+                                                      int(np.prod(shape))))
+            loaded[k] = a.reshape(shape).astype(dtype)
+        for k, a in loaded.items():
+            expect[k][0].copy_(torch.from_numpy(a))
         self.step = man['step']
         g = man['gate']
         self.gate = GateParams(g['t_lo'], g['t_hi'], g['p_lo'], g['p_hi'],

@@ -1,16 +1,16 @@
 """Measure emitter variants of the fused wdot kernels.
 
 Every layout and arithmetic choice of the generated kernels (where the
-per-species arrays live, the Kc form, the reaction order, the nu^T q
-contraction) is an emitter option; the defaults are the measured winners.
+per-species arrays live, the Kc form, the reaction order, the block size)
+is an emitter option; the defaults are the measured winners.
 This module builds a variant next to the default library and times both on
 the same states, scoring each against the default FP64 kernel with the
 conditioned error of create_rate_subs_amd.metrics:
 
     # on a machine with hipcc (no GPU needed): emit and compile
-    python -m create_rate_subs_amd.tune build mech53 mfma '{"contraction": "mfma"}'
+    python -m create_rate_subs_amd.tune build mech53 b128 '{"block": 128}'
     # on the GPU: time the default and the named variants, 2^20 states
-    python -m create_rate_subs_amd.tune run mech53 mfma
+    python -m create_rate_subs_amd.tune run mech53 b128
     # on the GPU: the default library against the batch size
     python -m create_rate_subs_amd.tune sweep mech53
 
@@ -24,9 +24,22 @@ import sys
 
 from . import build, hip_emitter, workloads
 
-__all__ = ['variant_dir', 'build_variant', 'run']
+__all__ = ['variant_dir', 'variant_lib', 'build_variant', 'run',
+           'PARITY_VARIANTS']
 
 TAU = 5e-2
+
+# Non-default layouts that the GPU parity suite checks against the oracle
+# like the default libraries (tests/test_gpu_block_option.py); build()
+# compiles them: (mechanism, variant name) -> emitter options.
+PARITY_VARIANTS = {
+    ('mech13', 'b128'): {'block': 128},
+    ('mech53', 'b128'): {'block': 128},
+}
+
+
+def variant_lib(mech, name):
+    return os.path.join(variant_dir(mech, name), 'libcrs_{}.so'.format(mech))
 
 
 def variant_dir(mech, name):

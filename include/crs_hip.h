@@ -146,10 +146,13 @@ int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
  * the integrator left as an INTEGRATOR macro): here the state never leaves
  * the device and the step is RK4.  work: 3 * NSP rows of stride ld
  * (caller-owned scratch).  Optional ignition bookkeeping as the reference's
- * IGN block (:259-266): with T0[n] and t_ign[n] (initialise t_ign < 0), a
- * state whose T first reaches T0 + dT_ign gets t_ign = the end time of that
- * step.  Enqueues 8 kernels per step on the stream (capturable in a graph;
- * no memset, no allocation).
+ * IGN block (:267-273): with T0[n] and t_ign[n] (initialise t_ign to NaN,
+ * the "not ignited" marker), a state whose T first reaches T0 + dT_ign at
+ * the end of a step gets t_ign = the time at the START of that step (the
+ * reference sets t_ign = t at :271 before its `t += h` at :276); step k
+ * starts at t0 plus k additions of h, as the reference accumulates t.
+ * Enqueues 8 kernels per step on the stream (capturable in a graph; no
+ * memset, no allocation).
  */
 int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
             double t0, double h, int nsteps, const double *T0, double *t_ign,

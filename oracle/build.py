@@ -28,7 +28,11 @@ from create_rate_subs_amd.create_rate_subs import load_mechanism  # noqa: E402
 from oracle import emit_c                                        # noqa: E402
 
 BUILD_ROOT = os.path.join(HERE, '_build')
-CFLAGS = ['-O3', '-fopenmp', '-fPIC', '-shared', '-std=gnu99']
+# -ffp-contract=off: no fused multiply-add anywhere (the default x86-64
+# target has no FMA instruction either), so every operation rounds as the
+# reference's C writes it
+CFLAGS = ['-O3', '-fopenmp', '-fPIC', '-shared', '-std=gnu99',
+          '-ffp-contract=off']
 
 
 def lib_path(name):
@@ -88,6 +92,8 @@ class Oracle(object):
                                                ctypes.c_int]
         self.lib.oracle_dydt.argtypes = [P, P, P, i64, i64, ctypes.c_int]
         self.lib.oracle_thermo.argtypes = [P, P, P, i64, i64, ctypes.c_int]
+        self.lib.oracle_time_wdot.argtypes = [P, P, P, P, i64, i64,
+                                              ctypes.c_int, ctypes.c_int, P]
 
     @staticmethod
     def default_threads():
@@ -101,6 +107,17 @@ class Oracle(object):
         self.lib.oracle_eval_wdot(_dptr(T), _dptr(P), _dptr(Y), _dptr(W), n,
                                   ld, nthreads or self.default_threads())
         return W
+
+    def time_wdot(self, T, P, Y, nthreads, reps=5):
+        """Wall seconds of ``reps`` timed passes over all states after one
+        warm-up pass (omp_get_wtime inside the library)."""
+        T, P, Y = _arr(T), _arr(P), _arr(Y)
+        W = np.empty((self.nsp, Y.shape[1]))
+        times = np.zeros(reps)
+        self.lib.oracle_time_wdot(_dptr(T), _dptr(P), _dptr(Y), _dptr(W),
+                                  T.size, Y.shape[1], nthreads, reps,
+                                  _dptr(times))
+        return times
 
     def eval_stages(self, T, P, Y, nthreads=None):
         """Every intermediate: dict of y_N, mw, rho, C, fwd, rev, pm, wdot."""

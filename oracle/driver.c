@@ -63,6 +63,24 @@ void oracle_eval_wdot(const double *T, const double *P, const double *Y,
 }
 
 /*
+ * CPU baseline timing: one untimed warm-up pass of oracle_eval_wdot, then
+ * `reps` timed passes; times_out[r] is pass r's wall time from
+ * omp_get_wtime (the reference's generated main times its loop with
+ * clock(), create_rate_subs.py:202, :285-302).
+ */
+void oracle_time_wdot(const double *T, const double *P, const double *Y,
+                      double *W, int64_t n, int64_t ld, int nthreads,
+                      int reps, double *times_out)
+{
+    oracle_eval_wdot(T, P, Y, W, n, ld, nthreads);
+    for (int r = 0; r < reps; ++r) {
+        const double t0 = omp_get_wtime();
+        oracle_eval_wdot(T, P, Y, W, n, ld, nthreads);
+        times_out[r] = omp_get_wtime() - t0;
+    }
+}
+
+/*
  * Every intermediate of the path for n states: y_N/mw/rho [n],
  * conc [NSP][ld], fwd [NR][ld], rev [NREV][ld], pres_mod [NPD][ld],
  * wdot [NSP][ld].  Any output pointer may be NULL.

@@ -115,7 +115,11 @@ def test_graph_replays_match_eager():
         del other
 
 
-def test_ignition_time_is_the_first_step_reaching_threshold():
+@pytest.mark.parametrize('t0', [0.0, -5.0 * H])
+def test_ignition_time_is_the_first_step_reaching_threshold(t0):
+    """The reference's IGN block (create_rate_subs.py:267-276) records t
+    before its t += h: a state over the threshold after the first step gets
+    t_ign = t0, and keeps it on later steps, negative t0 included."""
     name = 'mech53'
     P, y0 = _state(name, 64, 63)
     lib = hip_lib(name)
@@ -124,11 +128,12 @@ def test_ignition_time_is_the_first_step_reaching_threshold():
     # threshold already met for the first half, unreachable for the rest
     T0 = torch.tensor(np.where(np.arange(64) < 32, y0[0] - 1000.0,
                                y0[0] + 1.0e6), device='cuda')
-    t_ign = torch.full((64,), -1.0, dtype=torch.float64, device='cuda')
-    lib.rk4(Pd, yd, 0.0, h, 3, T0=T0, t_ign=t_ign, dT_ign=400.0)
+    t_ign = torch.full((64,), float('nan'), dtype=torch.float64,
+                       device='cuda')
+    lib.rk4(Pd, yd, t0, h, 3, T0=T0, t_ign=t_ign, dT_ign=400.0)
     ti = to_host(t_ign)
-    assert (ti[:32] == h).all()
-    assert (ti[32:] == -1.0).all()
+    assert (ti[:32] == t0).all()
+    assert np.isnan(ti[32:]).all()
 
 
 def test_harness_runs_the_reference_loop_on_device():

@@ -1,11 +1,15 @@
 """Surrogate kernels and pipeline on the GPU.
 
+Every test runs on the 13-species network and on the 53-species one of the
+learned-surrogate workload (55 -> 256 x 3 -> 53, GELU, on the 53-species /
+325-reaction mechanism):
+
 * the BF16-MFMA forward product and the full backward pass against the
   NumPy reference (oracle/mlp_ref.py) with the same BF16 roundings;
 * training by our kernels lowers the loss;
 * gated evaluation: every state the gate sends back is bit-identical to the
   exact FP64 kernel, out-of-box and invalid states are always sent back;
-* checkpoint round trip.
+* checkpoint round trip, and refusal of another mechanism's checkpoint.
 """
 
 import numpy as np
@@ -19,6 +23,8 @@ from oracle import mlp_ref
 from _helpers import hip_lib, mechanism
 
 pytestmark = pytest.mark.gpu
+
+MECHS = ['mech13', 'mech53']
 
 
 def _net(name='mech13', seed=3):
@@ -34,9 +40,11 @@ def _host_params(net):
     return W, b
 
 
+@pytest.mark.parametrize('name', MECHS)
 @pytest.mark.parametrize('M', [1, 100, 300])
-def test_forward_matches_reference(M):
-    net = _net()
+def test_forward_matches_reference(M, name):
+    net = _net(name)
+    assert net.nin == net.nsp + 2 and net.nout == net.nsp
     rng = np.random.default_rng(M)
     X = mlp_ref.bf16(rng.normal(size=(M, net.kin)))
     X[:, net.nin:] = 0.0
@@ -53,9 +61,10 @@ def test_forward_matches_reference(M):
     assert err < 5e-3, err
 
 
+@pytest.mark.parametrize('name', MECHS)
 @pytest.mark.parametrize('M', [64, 300])
-def test_gradients_match_cpu_oracle(M):
-    net = _net(seed=5)
+def test_gradients_match_cpu_oracle(M, name):
+    net = _net(name, seed=5)
     rng = np.random.default_rng(7 + M)
     X = mlp_ref.bf16(rng.normal(size=(M, net.kin)))
     X[:, net.nin:] = 0.0
@@ -95,9 +104,9 @@ def test_training_lowers_loss():
     assert hist[-1] < 0.25 * hist[0]
 
 
-def test_gate_fallback_is_exact_and_catches_invalid_states():
-    name = 'mech13'
-    net = _net(seed=13)
+@pytest.mark.parametrize('name', MECHS)
+def test_gate_fallback_is_exact_and_catches_invalid_states(name):
+    net = _net(name, seed=13)
     Td, Pd, Yd, W = _labelled(name, 1 << 14, 51)
     net.train(Td, Pd, Yd, W, epochs=2, batch=4096)
     net.gate.rmax = 1e-3
@@ -133,11 +142,12 @@ def _fit_random(name, n, seed):
     return net, Td, Pd, Yd
 
 
+@pytest.mark.parametrize('name', MECHS)
 @pytest.mark.parametrize('n', [5, 1000, 4096])
-def test_fused_inference_matches_layered(n):
+def test_fused_inference_matches_layered(n, name):
     """One-kernel inference (activations in registers, permuted tiles)
     against the layer-by-layer GEMM path, in the network's output space."""
-    net, Td, Pd, Yd = _fit_random('mech13', n, 60 + n)
+    net, Td, Pd, Yd = _fit_random(name, n, 60 + n)
     Wf = torch.empty((net.nsp, n), dtype=torch.float64, device='cuda')
     Wl = torch.empty_like(Wf)
     net.predict(Td, Pd, Yd, Wf, fused=True)
@@ -157,8 +167,8 @@ def test_fused_inference_matches_layered(n):
     assert d.max().item() < 3e-2 and d.mean().item() < 2e-3
 
 
-def test_fused_gate_flags_match_gate_kernel():
-    name = 'mech13'
+@pytest.mark.parametrize('name', MECHS)
+def test_fused_gate_flags_match_gate_kernel(name):
     net, Td, Pd, Yd = _fit_random(name, 3000, 77)
     Td[10] = 1.0e4                    # outside the training box
     Yd[2, 20] = -0.5                  # invalid composition
@@ -233,15 +243,44 @@ def test_trained_network_gain():
     print('calibration curve (normwise)', curve)
 
 
-def test_checkpoint_roundtrip(tmp_path):
-    net = _net(seed=17)
-    Td, Pd, Yd, W = _labelled('mech13', 1 << 13, 53)
+@pytest.mark.parametrize('name', MECHS)
+def test_checkpoint_roundtrip(tmp_path, name):
+    net = _net(name, seed=17)
+    Td, Pd, Yd, W = _labelled(name, 1 << 13, 53)
     net.train(Td, Pd, Yd, W, epochs=1, batch=4096)
     net.gate.rmax = 0.123
     net.save(str(tmp_path))
-    other = _net(seed=99)
+    other = _net(name, seed=99)
     other.load(str(tmp_path))
     a = net.predict(Td, Pd, Yd, torch.empty_like(W))
     b = other.predict(Td, Pd, Yd, torch.empty_like(W))
     assert torch.equal(a, b)
     assert other.gate.rmax == 0.123 and other.step == net.step
+
+
+def test_checkpoint_of_another_mechanism_is_refused(tmp_path):
+    """Same species count and network shape, different mechanism: mech13co
+    is mech13 with CO moved last, so only the hash tells them apart."""
+    import json
+    import os
+    net = _net('mech13', seed=17)
+    net.save(str(tmp_path))
+    other = _net('mech13co', seed=5)
+    assert other.shapes == net.shapes
+    before = other.p.clone()
+    with pytest.raises(ValueError, match='another mechanism'):
+        other.load(str(tmp_path))
+    assert torch.equal(other.p, before)
+    # an unexpected array name or shape in the manifest is refused too
+    man_path = os.path.join(str(tmp_path), 'manifest.json')
+    man = json.load(open(man_path))
+    man['files']['p']['shape'] = [3]
+    json.dump(man, open(man_path, 'w'))
+    same = _net('mech13', seed=1)
+    with pytest.raises(ValueError, match='array p'):
+        same.load(str(tmp_path))
+    man['files']['p']['shape'] = [net.nparam]
+    man['files']['extra'] = dict(file='p.bin', dtype='float32', shape=[1])
+    json.dump(man, open(man_path, 'w'))
+    with pytest.raises(ValueError, match='differ'):
+        same.load(str(tmp_path))

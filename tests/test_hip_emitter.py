@@ -158,8 +158,34 @@ def test_two_wave_choice(nsp, two):
     assert hip_emitter.resolve_opts(nsp, {'waves_per_simd': None})[
         'waves_per_simd'] is None
     # so is a layout the two-wave rows do not fit beside
-    for opts in ({'nasa': 'lds'}, {'g_store': 'lds'}, {'kc': 'species'}):
+    for opts in ({'nasa': 'lds'}, {'kc': 'species'}):
         assert hip_emitter.resolve_opts(nsp, opts)['waves_per_simd'] is None
+
+
+@pytest.mark.parametrize('opts', [{'g_store': 'lds'}, {'g_store': 'lds_sync'},
+                                  {'c_store': 'lds'}])
+def test_block_wide_lds_layouts_are_refused(opts):
+    """Block-wide LDS arrays would need a barrier after the per-state early
+    exits (partial blocks, the subset kernel): refused."""
+    with pytest.raises(ValueError):
+        hip_emitter.resolve_opts(53, opts)
+
+
+@pytest.mark.parametrize('nsp', [13, 53, 111])
+@pytest.mark.parametrize('block', [64, 128, 256, 512, 1024])
+def test_lds_rows_fit_the_block(nsp, block):
+    """The per-lane rows (and the NASA table) of every block resident at
+    the chosen occupancy fit the CU's 160 KB, whatever the block size."""
+    o = hip_emitter.resolve_opts(nsp, {'block': block})
+    wps = o['waves_per_simd'] or 1
+    wpb = block // 64
+    blocks = max(1, max(4 * wps, wpb) // wpb)
+    table = 128 * nsp if o['nasa'] == 'lds' else 0
+    used = blocks * (wpb * o['lds_species'] * 512 + table)
+    assert used <= 160 * 1024, (o, used)
+    if block == 256:
+        # the measured defaults
+        assert o['lds_species'] == {13: 78, 53: 39, 111: 78}[nsp]
 
 
 def test_wdot_block_option():
