@@ -757,7 +757,8 @@ __global__ __launch_bounds__(ITHREADS) void k_mlp_infer(
                 if (sc > 0.0 && fabs(r) > gp.rmax * sc)
                     bad = 1;
             }
-            flags[m] = bad;
+            if (flags)
+                flags[m] = bad;
         }
         // ---- fallback list: one atomic per wave ------------------------
         const uint64_t ballot = __ballot(valid && bad);
@@ -772,6 +773,14 @@ __global__ __launch_bounds__(ITHREADS) void k_mlp_infer(
                     (int32_t)m;
         }
     }
+}
+
+// *p = 0 (a fill kernel rather than a memset, so that a captured graph
+// replays it as a kernel node).
+__global__ void k_zero_i32(int32_t *__restrict__ p)
+{
+    if (threadIdx.x == 0)
+        *p = 0;
 }
 
 int check_gemm(const char *fn, int64_t M, int N, int64_t R)
@@ -978,16 +987,21 @@ int crs_mlp_pack_tiles(const void *W, int n_out, int n_in, int perm,
     return crs::launched(fn);
 }
 
-int crs_mlp_infer(const double *T, const double *P, const double *Y,
-                  int64_t n, int64_t ld, int nsp, int kin, int hidden,
-                  int layers, int kout, const void *tiles, const float *bias,
-                  const float *mu, const float *sd, const double *scale,
-                  const float *ymu, const float *ysd, double *W,
-                  crs_mlp_gate_params gp, const double *elem, int n_elem,
-                  const double *mw, int32_t *flags, int32_t *fb_idx,
-                  int32_t *fb_count, void *stream)
+}  // extern "C"
+
+namespace {
+
+// Checks and launch of k_mlp_infer, shared by crs_mlp_infer and the gated
+// entry points.  flags may be NULL when need_flags is false.
+int launch_infer(const char *fn, const double *T, const double *P,
+                 const double *Y, int64_t n, int64_t ld, int nsp, int kin,
+                 int hidden, int layers, int kout, const void *tiles,
+                 const float *bias, const float *mu, const float *sd,
+                 const double *scale, const float *ymu, const float *ysd,
+                 double *W, crs_mlp_gate_params gp, const double *elem,
+                 int n_elem, const double *mw, int32_t *flags, bool need_flags,
+                 int32_t *fb_idx, int32_t *fb_count, hipStream_t s)
 {
-    const char *fn = "crs_mlp_infer";
     if (kin != IKIN || hidden != IH || layers != 3 || kout != IKOUT)
         return crs::fail(CRS_ERR_UNSUPPORTED,
                          "%s: fused kernel is built for %d -> 3 x %d -> %d "
@@ -1001,17 +1015,81 @@ int crs_mlp_infer(const double *T, const double *P, const double *Y,
     if (n == 0)
         return CRS_OK;
     if (!T || !P || !Y || !tiles || !bias || !mu || !sd || !scale || !ymu ||
-        !ysd || !W || !mw || !flags || !fb_idx || !fb_count ||
-        (n_elem > 0 && !elem))
+        !ysd || !W || !mw || (need_flags && !flags) || !fb_idx ||
+        !fb_count || (n_elem > 0 && !elem))
         return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     if (((uintptr_t)tiles & 15) || ((uintptr_t)bias & 15))
         return crs::fail(CRS_ERR_ARG, "%s: tiles and bias need 16-byte "
                          "alignment", fn);
     const dim3 grid((unsigned)((n + ISTATES - 1) / ISTATES));
-    k_mlp_infer<<<grid, ITHREADS, 0, (hipStream_t)stream>>>(
+    k_mlp_infer<<<grid, ITHREADS, 0, s>>>(
         T, P, Y, n, ld, nsp, (const __bf16 *)tiles, bias, mu, sd, scale, ymu,
         ysd, W, gp, elem, n_elem, mw, flags, fb_idx, fb_count);
     return crs::launched(fn);
+}
+
+// Gated evaluation on one stream: count = 0, fused inference + gate
+// (appending fallback states to fb_idx and counting them in *fb_count),
+// then the mechanism library's exact subset kernel over fb_idx, bounded by
+// the device-side count.  Nothing is read back to the host.
+int gated(const char *fn, const crs_mlp_net *net, const double *T,
+          const double *P, const double *Y, double *W, int64_t n, int64_t ld,
+          crs_subset_fn exact, int exact_mode, int32_t *flags,
+          int32_t *fb_idx, int32_t *fb_count, hipStream_t s)
+{
+    if (!net || !exact)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL network or exact function",
+                         fn);
+    if (n < 0 || ld < n)
+        return crs::fail(CRS_ERR_ARG, "%s: bad sizes n=%lld ld=%lld", fn,
+                         (long long)n, (long long)ld);
+    if (n == 0)
+        return CRS_OK;
+    if (!fb_idx || !fb_count)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL fallback list", fn);
+    k_zero_i32<<<1, 64, 0, s>>>(fb_count);
+    CRS_TRY(crs::launched(fn));
+    CRS_TRY(launch_infer(fn, T, P, Y, n, ld, net->nsp, net->kin, net->hidden,
+                         net->layers, net->kout, net->tiles, net->bias,
+                         net->mu, net->sd, net->scale, net->ymu, net->ysd, W,
+                         net->gate, net->elem, net->n_elem, net->mw, flags,
+                         false, fb_idx, fb_count, s));
+    const int rc = exact(T, P, Y, W, fb_idx, fb_count, n, ld, exact_mode,
+                         (void *)s);
+    if (rc != CRS_OK)
+        return crs::fail(rc, "%s: exact fallback failed with code %d (the "
+                         "mechanism library's crs_last_error has the "
+                         "message)", fn, rc);
+    return CRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int crs_mlp_infer(const double *T, const double *P, const double *Y,
+                  int64_t n, int64_t ld, int nsp, int kin, int hidden,
+                  int layers, int kout, const void *tiles, const float *bias,
+                  const float *mu, const float *sd, const double *scale,
+                  const float *ymu, const float *ysd, double *W,
+                  crs_mlp_gate_params gp, const double *elem, int n_elem,
+                  const double *mw, int32_t *flags, int32_t *fb_idx,
+                  int32_t *fb_count, void *stream)
+{
+    return launch_infer("crs_mlp_infer", T, P, Y, n, ld, nsp, kin, hidden,
+                        layers, kout, tiles, bias, mu, sd, scale, ymu, ysd, W,
+                        gp, elem, n_elem, mw, flags, true, fb_idx, fb_count,
+                        (hipStream_t)stream);
+}
+
+int crs_mlp_eval_gated(const crs_mlp_net *net, const double *T,
+                       const double *P, const double *Y, double *W, int64_t n,
+                       int64_t ld, crs_subset_fn exact, int exact_mode,
+                       int32_t *flags, int32_t *fb_idx, int32_t *fb_count,
+                       void *stream)
+{
+    return gated("crs_mlp_eval_gated", net, T, P, Y, W, n, ld, exact,
+                 exact_mode, flags, fb_idx, fb_count, (hipStream_t)stream);
 }
 
 }  // extern "C"

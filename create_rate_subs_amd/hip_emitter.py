@@ -1837,20 +1837,25 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
 
 def _idx_kernel_tail(fill, call, preload):
     """Body of a fused kernel over a batch (idx == nullptr) or a subset:
-    the block's LDS table fill, then one state per thread.  With preload the
-    thread's T and P are loaded before the fill, so their latency overlaps
-    the fill's and its barrier instead of following them."""
+    the block's LDS table fill, then one state per thread.  The subset size
+    is m, or min(m, *m_dev) when the count lives on the device (m_dev not
+    null: a fallback list whose length no host code has read).  With
+    preload the thread's T and P are loaded before the fill, so their
+    latency overlaps the fill's and its barrier instead of following them.
+    The block-wide fill and its barrier come before any thread leaves."""
+    count = ('    const int64_t mm = m_dev ? (m < (int64_t)*m_dev ? m : '
+             '(int64_t)*m_dev) : m;\n')
     if not (preload and fill):
         return (fill +
                 '    const int64_t t = (int64_t)blockIdx.x * blockDim.x + '
-                'threadIdx.x;\n'
-                '    if (t >= m) return;\n'
+                'threadIdx.x;\n' + count +
+                '    if (t >= mm) return;\n'
                 '    const int64_t i = idx ? (int64_t)idx[t] : t;\n'
                 '    if (i < 0 || i >= ld) return;\n'
                 '    ' + call.format(pre='') + '\n}\n\n')
     return ('    const int64_t t = (int64_t)blockIdx.x * blockDim.x + '
-            'threadIdx.x;\n'
-            '    const int64_t i = t < m ? (idx ? (int64_t)idx[t] : t) : -1;\n'
+            'threadIdx.x;\n' + count +
+            '    const int64_t i = t < mm ? (idx ? (int64_t)idx[t] : t) : -1;\n'
             '    const bool live = i >= 0 && i < ld;\n'
             '    const double T_pre = live ? T_[i] : 0.0;\n'
             '    const double P_pre = live ? P_[i] : 0.0;\n' + fill +
@@ -2046,7 +2051,8 @@ def emit_hip_source(ir, opts=None):
         '    const double *__restrict__ K,\n'
         '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
         '    const double *__restrict__ Y, double *__restrict__ W,\n'
-        '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
+        '    const int32_t *__restrict__ idx, int64_t m,\n'
+        '    const int32_t *__restrict__ m_dev, int64_t ld)\n{\n' +
         _idx_kernel_tail(f64_fill,
                          'wdot_f64_state(K, T_, P_, Y, W, ld, i{pre});',
                          preload))
@@ -2141,7 +2147,8 @@ def emit_hip_source(ir, opts=None):
             '    const double *__restrict__ K,\n'
             '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
             '    const double *__restrict__ Y, double *__restrict__ W,\n'
-            '    const int32_t *__restrict__ idx, int64_t m, int64_t ld)\n{\n' +
+            '    const int32_t *__restrict__ idx, int64_t m,\n'
+            '    const int32_t *__restrict__ m_dev, int64_t ld)\n{\n' +
             _idx_kernel_tail(mixed_fill,
                              'wdot_mixed_state(K, T_, P_, Y, W, ld, i{pre});',
                              preload))
@@ -2167,12 +2174,12 @@ def _abi_text(ir, mixed=True):
     if mixed:
         launch_full = '''    if (mode == CRS_MODE_MIXED)
         k_wdot_mixed<<<crs::grid_for(n, kWdotBlock), kWdotBlock, 0, s>>>(
-            K, T, P, Y, wdot, nullptr, n, ld);
+            K, T, P, Y, wdot, nullptr, n, nullptr, ld);
     else
 '''
         launch_sub = '''    if (mode == CRS_MODE_MIXED)
         k_wdot_mixed<<<crs::grid_for(m, kWdotBlock), kWdotBlock, 0, s>>>(
-            K, T, P, Y, wdot, idx, m, ld);
+            K, T, P, Y, wdot, idx, m, m_dev, ld);
     else
 '''
         mixed_ok = 'mode == CRS_MODE_MIXED'
@@ -2323,7 +2330,7 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
 {launch_full}    k_wdot_f64<<<crs::grid_for(n, kWdotBlock), kWdotBlock, 0, s>>>(
-        K, T, P, Y, wdot, nullptr, n, ld);
+        K, T, P, Y, wdot, nullptr, n, nullptr, ld);
     return crs::launched(fn);
 }}
 
@@ -2383,11 +2390,13 @@ int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
     return crs::launched(fn);
 }}
 
-int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
-                         double *wdot, const int32_t *idx, int64_t m,
-                         int64_t ld, int mode, void *stream)
+// Shared by crs_eval_wdot_subset (host count m, m_dev NULL) and
+// crs_eval_wdot_subset_dev (at most m states, the count read on the device).
+static int wdot_subset(const char *fn, const double *T, const double *P,
+                       const double *Y, double *wdot, const int32_t *idx,
+                       int64_t m, const int32_t *m_dev, int64_t ld, int mode,
+                       void *stream)
 {{
-    const char *fn = "crs_eval_wdot_subset";
     if (m < 0)
         return crs::fail(CRS_ERR_ARG, "%s: m must be >= 0", fn);
     if (ld < 0)
@@ -2403,8 +2412,27 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
 {launch_sub}    k_wdot_f64<<<crs::grid_for(m, kWdotBlock), kWdotBlock, 0, s>>>(
-        K, T, P, Y, wdot, idx, m, ld);
+        K, T, P, Y, wdot, idx, m, m_dev, ld);
     return crs::launched(fn);
+}}
+
+int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
+                         double *wdot, const int32_t *idx, int64_t m,
+                         int64_t ld, int mode, void *stream)
+{{
+    return wdot_subset("crs_eval_wdot_subset", T, P, Y, wdot, idx, m,
+                       nullptr, ld, mode, stream);
+}}
+
+int crs_eval_wdot_subset_dev(const double *T, const double *P,
+                             const double *Y, double *wdot,
+                             const int32_t *idx, const int32_t *count,
+                             int64_t m_max, int64_t ld, int mode, void *stream)
+{{
+    const char *fn = "crs_eval_wdot_subset_dev";
+    CRS_TRY(crs::check_ptr(fn, count, "count", m_max));
+    return wdot_subset(fn, T, P, Y, wdot, idx, m_max, count, ld, mode,
+                       stream);
 }}
 
 }}  // extern "C"

@@ -23,7 +23,8 @@ MODES = {'fp64': 0, 'mixed': 1}
 EXPORTED_SYMBOLS = ('crs_get_mech_info', 'crs_last_error', 'crs_eval_conc',
                     'crs_eval_rxn_rates', 'crs_get_rxn_pres_mod',
                     'crs_eval_spec_rates', 'crs_eval_wdot',
-                    'crs_eval_wdot_subset', 'crs_dydt', 'crs_rk4')
+                    'crs_eval_wdot_subset', 'crs_eval_wdot_subset_dev',
+                    'crs_dydt', 'crs_rk4')
 
 
 class CrsError(RuntimeError):
@@ -48,6 +49,7 @@ _SIGS = {
     'crs_eval_spec_rates': [_P] * 5 + [_I64, _I64, _P],
     'crs_eval_wdot': [_P] * 4 + [_I64, _I64, ctypes.c_int, _P],
     'crs_eval_wdot_subset': [_P] * 5 + [_I64, _I64, ctypes.c_int, _P],
+    'crs_eval_wdot_subset_dev': [_P] * 6 + [_I64, _I64, ctypes.c_int, _P],
     'crs_dydt': [_P] * 3 + [_I64, _I64, ctypes.c_int, _P],
     'crs_rk4': [_P] * 3 + [_I64, _I64, ctypes.c_double, ctypes.c_double,
                            ctypes.c_int, _P, _P, ctypes.c_double, _P],
@@ -222,6 +224,23 @@ class MechLib(object):
                                      _ptr(T0), _ptr(t_ign), float(dT_ign),
                                      _stream()))
         return y
+
+    def eval_wdot_subset_dev(self, T, P, Y, W, idx, count, mode='fp64'):
+        """States idx[t], t < min(count[0], idx.numel()); the count is a
+        device int32 tensor that is never read on the host."""
+        n = T.numel()
+        ld = self._rows(Y, self.nsp, n, 'Y')
+        if self._rows(W, self.nsp, n, 'wdot') != ld:
+            raise ValueError('wdot and Y must share the row stride')
+        for t, what in ((idx, 'idx'), (count, 'count')):
+            if not (t.is_cuda and t.dtype == torch.int32 and t.dim() == 1
+                    and t.is_contiguous()):
+                raise ValueError(what + ' must be a contiguous cuda int32 '
+                                 'vector')
+        self._check(self.lib.crs_eval_wdot_subset_dev(
+            _ptr(T), _ptr(P), _ptr(Y), _ptr(W), _ptr(idx), _ptr(count),
+            idx.numel(), ld, MODES[mode], _stream()))
+        return W
 
     def eval_wdot_subset(self, T, P, Y, W, idx, mode='fp64'):
         n = T.numel()

@@ -47,6 +47,18 @@ class GateParams(ctypes.Structure):
                 ('rmax', ctypes.c_double)]
 
 
+class NetDesc(ctypes.Structure):
+    """crs_mlp_net of include/crs_mlp.h."""
+    _fields_ = [('nsp', ctypes.c_int), ('kin', ctypes.c_int),
+                ('hidden', ctypes.c_int), ('layers', ctypes.c_int),
+                ('kout', ctypes.c_int), ('tiles', ctypes.c_void_p),
+                ('bias', ctypes.c_void_p), ('mu', ctypes.c_void_p),
+                ('sd', ctypes.c_void_p), ('scale', ctypes.c_void_p),
+                ('ymu', ctypes.c_void_p), ('ysd', ctypes.c_void_p),
+                ('gate', GateParams), ('elem', ctypes.c_void_p),
+                ('n_elem', ctypes.c_int), ('mw', ctypes.c_void_p)]
+
+
 class MlpError(RuntimeError):
     pass
 
@@ -83,6 +95,8 @@ class MlpLib(object):
         'crs_mlp_infer': [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _I, _P, _P,
                           _P, _P, _P, _P, _P, _P, GateParams, _P, _I, _P, _P,
                           _P, _P, _P],
+        'crs_mlp_eval_gated': [ctypes.POINTER(NetDesc), _P, _P, _P, _P, _I64,
+                               _I64, _P, _I, _P, _P, _P, _P],
     }
 
     def __init__(self, path=MLP_LIB):
@@ -445,6 +459,22 @@ class Surrogate(object):
                  _s())
         return W
 
+    def net_desc(self):
+        """The network as the C ABI's crs_mlp_net (weights packed first);
+        the tensors it points to stay alive as attributes of self."""
+        tiles, bias = self.pack()
+        return NetDesc(self.nsp, self.kin, self.hid, len(self.shapes) - 1,
+                       self.kout, _p(tiles), _p(bias), _p(self.mu),
+                       _p(self.sd), _p(self.scale), _p(self.ymu),
+                       _p(self.ysd), self.gate, _p(self.elem),
+                       self.elem.shape[0], _p(self.mw))
+
+    def exact_fn(self):
+        """Address of the mechanism library's crs_eval_wdot_subset_dev (the
+        crs_subset_fn the gated entry points call)."""
+        return ctypes.cast(self.mech.lib.crs_eval_wdot_subset_dev,
+                           ctypes.c_void_p)
+
     def gate_states(self, T, P, Y, W):
         n = T.numel()
         flags = self._buf('flags', (n,), torch.int32)
@@ -455,17 +485,33 @@ class Surrogate(object):
                  _p(self.mw), _p(flags), _p(idx), _p(cnt), _s())
         return flags, idx, cnt
 
-    def evaluate(self, T, P, Y, W, mode='fp64', fused=True):
-        """Gated surrogate: predict, gate, exact fallback.  Returns m."""
+    def evaluate(self, T, P, Y, W, mode='fp64', fused=True, read_count=True):
+        """Gated surrogate: predict, gate, exact fallback.
+
+        With the fused network shape this is one call of the C ABI's
+        crs_mlp_eval_gated (inference, gate and the mechanism library's
+        exact subset kernel on the current stream, the fallback count kept
+        on the device).  Returns the fallback count m, read back after the
+        work is enqueued, or with read_count=False the device count tensor
+        (no synchronisation at all).
+        """
+        from .runtime import MODES
+        n = T.numel()
         if fused and self.fused_ok:
-            flags, idx, cnt = self.infer(T, P, Y, W)
-        else:
-            self.predict(T, P, Y, W, fused=False)
-            flags, idx, cnt = self.gate_states(T, P, Y, W)
+            flags = self._buf('flags', (n,), torch.int32)
+            idx = self._buf('fb_idx', (n,), torch.int32)
+            cnt = self._buf('fb_count', (1,), torch.int32)
+            desc = self.net_desc()
+            self.mlp('crs_mlp_eval_gated', ctypes.byref(desc), _p(T), _p(P),
+                     _p(Y), _p(W), n, Y.stride(0), self.exact_fn(),
+                     MODES[mode], _p(flags), _p(idx), _p(cnt), _s())
+            return int(cnt.item()) if read_count else cnt
+        self.predict(T, P, Y, W, fused=False)
+        flags, idx, cnt = self.gate_states(T, P, Y, W)
         m = int(cnt.item())
         if m:
             self.mech.eval_wdot_subset(T, P, Y, W, idx[:m], mode)
-        return m
+        return m if read_count else cnt
 
     def gross_scale(self, T, P, Y):
         """S_k of the states, from the mechanism's FP64 stage kernels."""

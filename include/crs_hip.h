@@ -123,6 +123,22 @@ int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
                          int64_t ld, int mode, void *stream);
 
 /*
+ * crs_eval_wdot_subset with the subset's length in device memory: states
+ * idx[t] for t < min(*count, m_max), where count is a device int32 written
+ * by earlier work on the same stream (the surrogate's fallback list), so
+ * no host code has to read it.  The launch covers m_max states; those
+ * beyond *count leave at once.  The signature is crs_subset_fn of
+ * include/crs_mlp.h: the gated surrogate calls it for its exact fallback.
+ * No counterpart in the reference (its states are all evaluated every
+ * call).
+ */
+int crs_eval_wdot_subset_dev(const double *T, const double *P,
+                             const double *Y, double *wdot,
+                             const int32_t *idx, const int32_t *count,
+                             int64_t m_max, int64_t ld, int mode,
+                             void *stream);
+
+/*
  * Batched constant-pressure right-hand side.  Replaces `void dydt(const
  * double t, const double pres, const double *y, double *dy)` emitted by
  * rate_subs.py:1829-1946 (CONP branch; prototype rate_subs.py:1808-1816):

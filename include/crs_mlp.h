@@ -120,6 +120,47 @@ int crs_mlp_infer(const double *T, const double *P, const double *Y,
                   const double *mw, int32_t *flags, int32_t *fb_idx,
                   int32_t *fb_count, void *stream);
 
+/* The exact evaluation a gated call falls back to: crs_eval_wdot_subset_dev
+ * of a mechanism library (include/crs_hip.h), which evaluates states idx[t]
+ * for t < min(*count, m_max) and reads the count on the device. */
+typedef int (*crs_subset_fn)(const double *T, const double *P,
+                             const double *Y, double *wdot,
+                             const int32_t *idx, const int32_t *count,
+                             int64_t m_max, int64_t ld, int mode,
+                             void *stream);
+
+/* A trained network for the gated entry points: the arguments of
+ * crs_mlp_infer after the state arrays, gathered (device pointers; the
+ * struct itself is read on the host during the call only). */
+typedef struct crs_mlp_net {
+    int nsp, kin, hidden, layers, kout;
+    const void *tiles;   /* crs_mlp_pack_tiles order, 16-byte aligned */
+    const float *bias;   /* 256 + 256 + 256 + 64, 16-byte aligned */
+    const float *mu, *sd;            /* feature normalisation [nsp + 2] */
+    const double *scale;             /* s_k [nsp] */
+    const float *ymu, *ysd;          /* output normalisation [nsp] */
+    crs_mlp_gate_params gate;
+    const double *elem;              /* [n_elem][nsp] */
+    int n_elem;
+    const double *mw;                /* [nsp] */
+} crs_mlp_net;
+
+/* Gated surrogate evaluation of n states into W [nsp][ld]: fused inference
+ * and gate (crs_mlp_infer), then `exact` over the states the gate sent back,
+ * all enqueued on `stream` with no host synchronisation (the fallback count
+ * stays on the device).  W then holds the surrogate's rates for admitted
+ * states and the exact kernel's for every other state (bit-identical to
+ * crs_eval_wdot in mode exact_mode).  Caller-owned workspace: fb_idx [n]
+ * (the fallback states, in no particular order) and *fb_count (their
+ * number, zeroed by this call); flags [n] (1 = sent back) may be NULL.
+ * This is the reference's eval_wdot chain (rate_subs.py:1845-1880) with a
+ * learned evaluation in front of it; the reference has no surrogate. */
+int crs_mlp_eval_gated(const crs_mlp_net *net, const double *T,
+                       const double *P, const double *Y, double *W, int64_t n,
+                       int64_t ld, crs_subset_fn exact, int exact_mode,
+                       int32_t *flags, int32_t *fb_idx, int32_t *fb_count,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,29 @@ def test_wdot_subset(name):
 
 
 @pytest.mark.parametrize('mode', ['fp64', 'mixed'])
+@pytest.mark.parametrize('count', [0, 3, 5, 9])
+def test_wdot_subset_device_count(mode, count):
+    """crs_eval_wdot_subset_dev: the subset's length is read on the device
+    and capped by the list's length m_max = 5."""
+    import torch
+    name = 'mech53'
+    T, P, Y = _states(name, 5000, 15)
+    lib = hip_lib(name)
+    Td, Pd, Yd = to_dev(T, P, Y)
+    full = to_host(lib.eval_wdot(Td, Pd, Yd, mode))
+    order = [4999, 0, 17, 2500, 3]
+    idx = torch.tensor(order, dtype=torch.int32, device='cuda')
+    cnt = torch.tensor([count], dtype=torch.int32, device='cuda')
+    W = torch.full_like(Yd, np.nan)
+    lib.eval_wdot_subset_dev(Td, Pd, Yd, W, idx, cnt, mode)
+    Wh = to_host(W)
+    sel = sorted(order[:min(count, 5)])
+    np.testing.assert_array_equal(Wh[:, sel], full[:, sel])
+    rest = np.setdiff1d(np.arange(5000), sel)
+    assert np.isnan(Wh[:, rest]).all()
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
 def test_wdot_ragged_leading_dimension(mode):
     """Rows with a stride larger than the state count (views into wider
     buffers): the same rates as the compact layout, padding untouched."""

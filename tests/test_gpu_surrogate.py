@@ -130,6 +130,52 @@ def test_gate_fallback_is_exact_and_catches_invalid_states(name):
     assert audit['states'] == 250
 
 
+@pytest.mark.parametrize('name', MECHS)
+@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
+def test_gated_c_abi_matches_its_parts(name, mode):
+    """crs_mlp_eval_gated (inference, gate and the mechanism library's
+    crs_eval_wdot_subset_dev on one stream, the count never read on the
+    host): states it sends back carry the exact kernel's rates bit for bit,
+    the others the fused surrogate's, and the device count is the number of
+    flags.  Also replayed from a captured HIP graph, with other work between
+    replays, against the eager result."""
+    net = _net(name, seed=23)
+    Td, Pd, Yd, W = _labelled(name, 1 << 14, 81)
+    net.train(Td, Pd, Yd, W, epochs=1, batch=4096)
+    T2, P2, Y2, _w = _labelled(name, 3000, 82)
+    T2[5] = 9000.0                          # outside the training box
+    # admit about half of the states: rmax at the median residual
+    Wsur = torch.empty_like(_w)
+    net.predict(T2, P2, Y2, Wsur)
+    net.gate.rmax = float(net.residuals(Wsur).median().item())
+    Wex = hip_lib(name).eval_wdot(T2, P2, Y2, mode)
+    out = torch.full_like(Wex, float('nan'))
+    cnt = net.evaluate(T2, P2, Y2, out, mode=mode, read_count=False)
+    flags = net._bufs['flags'][:3000].bool()
+    m = int(cnt.item())
+    print(name, mode, 'fallback', m, 'of 3000')
+    assert m == int(flags.sum().item()) and 0 < m < 3000
+    assert bool(flags[5])
+    assert torch.equal(out[:, flags], Wex[:, flags])
+    assert torch.equal(out[:, ~flags], Wsur[:, ~flags])
+    # graph capture of the same call: no host read, no memset node
+    g_out = torch.full_like(Wex, float('nan'))
+    net.evaluate(T2, P2, Y2, g_out, mode=mode, read_count=False)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        net.evaluate(T2, P2, Y2, g_out, mode=mode, read_count=False)
+    for rep in range(3):
+        g_out.fill_(float('nan'))
+        other = hip_lib(name).eval_wdot(T2, P2, Y2, 'fp64')
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(g_out, out), rep
+        assert int(net._bufs['fb_count'][0].item()) == m
+        del other
+
+
 def _fit_random(name, n, seed):
     """Network with normalisation fitted to n labelled states, no training."""
     net = _net(name, seed=seed)
