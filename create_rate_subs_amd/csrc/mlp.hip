@@ -775,6 +775,9 @@ __global__ __launch_bounds__(ITHREADS) void k_mlp_infer(
     }
 }
 
+// Most states one call of crs_mlp_eval_gated_noalloc takes.
+constexpr int64_t kNoAllocStates = 1 << 22;
+
 // *p = 0 (a fill kernel rather than a memset, so that a captured graph
 // replays it as a kernel node).
 __global__ void k_zero_i32(int32_t *__restrict__ p)
@@ -1065,7 +1068,60 @@ int gated(const char *fn, const crs_mlp_net *net, const double *T,
 
 }  // namespace
 
+// Fallback list of crs_mlp_eval_gated_noalloc: the count in element 0
+// (elements 1..63 pad the list to a 256-byte boundary), then up to
+// kNoAllocStates state indices.  16.8 MB on every GPU the library runs
+// on, held until the process ends.  Host code never uses this name as an
+// address (on the host it is only a stand-in): it uses the per-device
+// address scratch_address() takes from hipGetSymbolAddress.
+static __device__ __align__(256) int32_t scratch[64 + kNoAllocStates];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy at its own address).
+int scratch_address(const char *fn, int32_t **out)
+{
+    static std::mutex lock;
+    static int32_t *addr[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return crs::fail(CRS_ERR_HIP, "%s: no current HIP device", fn);
+    std::lock_guard<std::mutex> g(lock);
+    if (addr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return crs::fail(CRS_ERR_HIP, "%s: static fallback list: %s",
+                             fn, hipGetErrorString(e));
+        addr[dev] = (int32_t *)p;
+    }
+    *out = addr[dev];
+    return CRS_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int crs_mlp_eval_gated_noalloc(const crs_mlp_net *net, const double *T,
+                               const double *P, const double *Y, double *W,
+                               int64_t n, int64_t ld, crs_subset_fn exact,
+                               int exact_mode, void *stream)
+{
+    const char *fn = "crs_mlp_eval_gated_noalloc";
+    if (n > kNoAllocStates)
+        return crs::fail(CRS_ERR_ARG, "%s: n = %lld exceeds the static "
+                         "fallback list (%lld states)", fn, (long long)n,
+                         (long long)kNoAllocStates);
+    if (n <= 0)
+        return gated(fn, net, T, P, Y, W, n, ld, exact, exact_mode, nullptr,
+                     nullptr, nullptr, (hipStream_t)stream);
+    int32_t *list = nullptr;
+    CRS_TRY(scratch_address(fn, &list));
+    return gated(fn, net, T, P, Y, W, n, ld, exact, exact_mode, nullptr,
+                 list + 64, list, (hipStream_t)stream);
+}
 
 int crs_mlp_infer(const double *T, const double *P, const double *Y,
                   int64_t n, int64_t ld, int nsp, int kin, int hidden,

@@ -161,6 +161,21 @@ int crs_mlp_eval_gated(const crs_mlp_net *net, const double *T,
                        int32_t *flags, int32_t *fb_idx, int32_t *fb_count,
                        void *stream);
 
+/* crs_mlp_eval_gated without caller workspace and without allocating:
+ * the fallback list and its count live in a static device array of this
+ * library (16.8 MB per GPU, held until the process ends), so the call makes
+ * no hipMalloc/hipFree and no allocating library call; n is at most
+ * 4194304 (2^22) states, larger n is refused with CRS_ERR_ARG before any
+ * work is enqueued.  No flags are written.  Every call on one GPU shares
+ * that GPU's list (the count is cleared by each call): calls that overlap
+ * on one GPU (other streams or threads) race.  No counterpart in the
+ * reference; for callers that cannot allocate (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). */
+int crs_mlp_eval_gated_noalloc(const crs_mlp_net *net, const double *T,
+                               const double *P, const double *Y, double *W,
+                               int64_t n, int64_t ld, crs_subset_fn exact,
+                               int exact_mode, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
