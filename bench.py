@@ -362,6 +362,10 @@ def main():
         lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd)
         if lib.modes & 2:
             lib.eval_wdot(Td, Pd, Yd, 'mixed', out=Wd)
+        # one CONP dydt launch on the same states (row 0 = T)
+        yd = torch.cat([Td[None], Yd[:nsp - 1]]).contiguous()
+        lib.dydt(Pd, yd, out=torch.empty_like(yd))
+        del yd
         sur = None
         if nsp + 2 <= 64:
             # one fused surrogate launch (untrained weights: traffic and

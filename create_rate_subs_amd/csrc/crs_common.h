@@ -19,6 +19,10 @@ namespace crs {
 // Threads per block for the per-state kernels: four wave64 wavefronts.
 constexpr int kBlock = 256;
 
+// Most states one launch of a fused kernel addresses: a state's byte offset
+// within a row, 8 x index, is a 32-bit value (ld_row / st_row below).
+constexpr int64_t kMaxChunk = (int64_t)1 << 29;
+
 inline char *error_buffer()
 {
     static thread_local char buf[512] = {0};
@@ -132,6 +136,82 @@ __device__ __forceinline__ double exp2_64(double t)
     return __builtin_ldexp(p, (int)n);
 }
 
+// The constants of exp64 / exp2_64 as run-time values held in registers.
+// A fused kernel takes them once at its start (ExpK k = exp_consts()) and
+// passes them to exp64k / exp2_64k: each reaction block is its own basic
+// block, and with literal coefficients every block would materialise all of
+// them again (two scalar moves per constant, plus a vector move for the one
+// that must sit in a VGPR: a VALU instruction takes one scalar operand).
+// The values pass through empty asm so the compiler cannot fold them back
+// into literals; c8 is kept in a VGPR pair for the first Horner step.
+struct ExpK {
+    double c9, c8, c7, c6, c5, c4, c3, c2, c1, c0;
+    double ln2, ln2lo, log2e;
+};
+
+__device__ __forceinline__ double sgpr_const(double x)
+{
+    asm("; const" : "+s"(x));
+    return x;
+}
+
+__device__ __forceinline__ double vgpr_const(double x)
+{
+    asm("; const" : "+v"(x));
+    return x;
+}
+
+__device__ __forceinline__ ExpK exp_consts()
+{
+    ExpK k;
+    k.c9 = sgpr_const(2.510038549551032e-08);
+    k.c8 = vgpr_const(2.7620088445409746e-07);
+    k.c7 = sgpr_const(2.7557268459997064e-06);
+    k.c6 = sgpr_const(2.4801521295954376e-05);
+    k.c5 = sgpr_const(0.00019841269863053618);
+    k.c4 = sgpr_const(0.0013888888917213717);
+    k.c3 = sgpr_const(0.008333333333330062);
+    k.c2 = sgpr_const(0.04166666666662413);
+    k.c1 = sgpr_const(0.16666666666666669);
+    k.c0 = sgpr_const(0.5000000000000001);
+    k.ln2 = sgpr_const(0.6931471805599453);
+    k.ln2lo = sgpr_const(2.3190468138462996e-17);
+    k.log2e = sgpr_const(1.4426950408889634);
+    return k;
+}
+
+// exp64 / exp2_64 on the register-held constants: the same operations in
+// the same order, so the same results bit for bit.
+__device__ __forceinline__ double exp_tail(const ExpK &k, double r, double n)
+{
+    double q = __builtin_fma(k.c9, r, k.c8);
+    q = __builtin_fma(q, r, k.c7);
+    q = __builtin_fma(q, r, k.c6);
+    q = __builtin_fma(q, r, k.c5);
+    q = __builtin_fma(q, r, k.c4);
+    q = __builtin_fma(q, r, k.c3);
+    q = __builtin_fma(q, r, k.c2);
+    q = __builtin_fma(q, r, k.c1);
+    q = __builtin_fma(q, r, k.c0);
+    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
+__device__ __forceinline__ double exp64k(const ExpK &k, double x)
+{
+    const double n = __builtin_rint(x * k.log2e);
+    double r = __builtin_fma(-n, k.ln2, x);
+    r = __builtin_fma(-n, k.ln2lo, r);
+    return exp_tail(k, r, n);
+}
+
+__device__ __forceinline__ double exp2_64k(const ExpK &k, double t)
+{
+    const double n = __builtin_rint(t);
+    const double r = (t - n) * k.ln2;
+    return exp_tail(k, r, n);
+}
+
 // exp2_64 / exp64 with q(r) by Estrin's scheme instead of Horner's: the
 // same coefficients, 3 more multiplies, a dependency chain of 5 operations
 // instead of 10 (1.20 ulp max, as Horner, by exact emulation in
@@ -240,6 +320,59 @@ __device__ __forceinline__ CRS_LDS double *opaque_lds(CRS_LDS double *p)
 {
     asm volatile("" : "+v"(p));
     return p;
+}
+
+// Row access of the [rows][ld] arrays with a uniform row pointer and a
+// 32-bit per-lane byte offset: the address is (row in SGPRs) + zext(off),
+// which the hardware takes directly (global_load/store with an SGPR base),
+// instead of one 64-bit VGPR address per row.  off = 8 * state index.
+__device__ __forceinline__ double ld_row(const double *__restrict__ row,
+                                         uint32_t off)
+{
+    return *(const double *)((const char *)row + off);
+}
+
+__device__ __forceinline__ void st_row(double *__restrict__ row, uint32_t off,
+                                       double v)
+{
+    *(double *)((char *)row + off) = v;
+}
+
+// A copy of x the compiler cannot prove equal to x or to any copy with
+// another Tag (the tag is an operand of the empty asm).  Each reaction block
+// of the fused kernels reads its per-state inputs through its own copies, so
+// equal products in two blocks (H_k H_k, G_k H_k, b log2 T) are not merged
+// into one value kept live between them.  No instruction is emitted.
+template <int Tag>
+__device__ __forceinline__ double fresh(double x)
+{
+    asm("; fresh %1" : "+v"(x) : "n"(Tag));
+    return x;
+}
+
+// The same for a wave-uniform integer held in SGPRs (a row stride): the
+// row address of a late store is then formed where it is used instead of
+// being shared with an early load's and kept in SGPRs in between.
+template <int Tag>
+__device__ __forceinline__ int64_t fresh_s(int64_t x)
+{
+    asm("; fresh %1" : "+s"(x) : "n"(Tag));
+    return x;
+}
+
+// A wave-uniform true the compiler cannot see through, one per Tag.  The
+// fused kernels open every reaction block with `if (crs::go<N>())`: the
+// block becomes its own basic block, so the code generator materialises the
+// block's literal constants inside it instead of sharing equal 32-bit halves
+// of constants across the whole straight-line kernel (kept in SGPRs from
+// the first use to the last, they were spilled to VGPR lanes).  Costs one
+// scalar move, compare and branch per block.
+template <int Tag>
+__device__ __forceinline__ bool go()
+{
+    int v;
+    asm("s_mov_b32 %0, 1 ; go %1" : "=s"(v) : "n"(Tag));
+    return v != 0;
 }
 
 // s/R - h/RT of one species from its 7 coefficients (in LDS) on the basis

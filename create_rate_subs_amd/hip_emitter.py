@@ -912,15 +912,17 @@ def _fast_arr(a, K, name, expfn='exp', log2=False):
         return ['const double {} = {};'.format(name, _arr_expr(a, K))], None
     sc = _LOG2E if log2 else 1.0
     lt = 'log2T' if log2 else 'logT'
-    terms = []
-    if a.logA is not None:
-        terms.append(K(a.logA * sc))
+    # the T-dependent terms first and ln A added last: every instruction
+    # then takes one constant (a VALU operation reads one scalar operand;
+    # fma(b, log T, ln A) would first copy ln A into a VGPR pair)
+    x = ''
     if a.b is not None:
-        terms.append('{} * {}'.format(K(a.b), lt))
-    x = ' + '.join(terms)
+        x = '{} * {}'.format(K(a.b), lt)
     if a.E is not None:
         x = ('{} - {} * iT'.format(x, K(a.E * sc)) if x else
              '-{} * iT'.format(K(a.E * sc)))
+    if a.logA is not None:
+        x = '({}) + {}'.format(x, K(a.logA * sc)) if x else K(a.logA * sc)
     out = ['const double x_{} = {};'.format(name, x)]
     if a.pre is not None:
         out.append('const double {n} = {p} * {e}(x_{n});'.format(
@@ -1832,7 +1834,157 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     if opts.get('exp_poly') == 'estrin':
         lines = [l.replace('crs::exp2_64(', 'crs::exp2_64_estrin(')
                   .replace('crs::exp64(', 'crs::exp64_estrin(') for l in lines]
+    if opts.get('launder', False):
+        lines = _launder_blocks(lines)
+    if opts.get('row_offsets', True):
+        lines = _row_offsets(lines)
+    split = opts.get('split_blocks', 32)
+    if split:
+        lines = _split_blocks(lines, int(split))
     return lines
+
+
+_DECL = re.compile(r'^(const double|CRS_LDS double \*const) (\w+) = ')
+
+
+def _split_blocks(lines, per):
+    """Make every ``per`` consecutive reaction blocks one basic block.
+
+    Straight-line code is one basic block, in which the code generator
+    materialises each distinct 32-bit half of a literal once and keeps it
+    in an SGPR until its last use anywhere in the kernel; at 53 species
+    that spilled ~350 SGPRs into VGPR lanes.  Wrapping a group of blocks in
+    `if (crs::go<N>()) { ... }` (a uniform true the compiler cannot see
+    through: a scalar branch that is always taken) confines that sharing to
+    the group.  Values a group declares for a later one (LDS reads issued
+    ahead) are declared before the group and assigned inside it.  The
+    exponential's constants are taken once, in registers (crs::ExpK),
+    instead of per group.
+    """
+    lines = ['const crs::ExpK ek = crs::exp_consts();'] + [
+        l.replace('crs::exp2_64(', 'crs::exp2_64k(ek, ')
+         .replace('crs::exp64(', 'crs::exp64k(ek, ') for l in lines]
+    # segments: ('plain', [lines]) outside groups, ('group', [lines])
+    segs = []
+    cur = None
+    count = 0
+    for l in lines:
+        if l.startswith('{  // rxn'):
+            if count % per == 0:
+                cur = []
+                segs.append(('group', cur))
+            count += 1
+            cur.append(l)
+        elif cur is not None and (l.startswith((' ', '}', 'CRS_LDS',
+                                                'const double gl',
+                                                'crs::st_row')) or
+                                  l.startswith('if (')):
+            cur.append(l)
+        else:
+            if segs and segs[-1][0] == 'plain':
+                segs[-1][1].append(l)
+            else:
+                segs.append(('plain', [l]))
+            cur = None
+            count = 0
+    out = []
+    groups = 0
+    for n, (kind, body) in enumerate(segs):
+        if kind == 'plain':
+            out += body
+            continue
+        groups += 1
+        later = '\n'.join(l for _k, b in segs[n + 1:] for l in b)
+        hoist = []
+        inner = []
+        for l in body:
+            mo = _DECL.match(l)
+            if mo and re.search(r'\b{}\b'.format(mo.group(2)), later):
+                hoist.append(('double' if mo.group(1) == 'const double'
+                              else 'CRS_LDS double *', mo.group(2)))
+                l = l.replace('{} {} ='.format(mo.group(1), mo.group(2)),
+                              '{} ='.format(mo.group(2)), 1)
+            inner.append(l)
+        out += ['{} {};'.format(t, h) for t, h in hoist]
+        out.append('if (crs::go<{}>()) {{'.format(groups))
+        out += inner
+        out.append('}')
+    return out
+
+
+_ROW_LOAD = re.compile(r'\b(Y|T_|P_)\[(?:(\d+) \* ld \+ )?i\]')
+_ROW_STORE = re.compile(r'^(\s*)W\[(\d+) \* ld \+ i\] = (.*);$')
+
+
+def _row_offsets(lines):
+    """Rewrite the fused kernels' per-state global accesses as uniform row
+    pointers plus one 32-bit lane offset (crs::ld_row / st_row): every row
+    address is then an SGPR base and the shared offset io = 8 i, instead of
+    a 64-bit VGPR pair per row.  Needs 8 i < 2^32 (ld below 2^29, checked
+    by the entry points)."""
+    def load(mo):
+        row = mo.group(1) if mo.group(2) is None else '{} + {} * ld'.format(
+            mo.group(1), mo.group(2))
+        return 'crs::ld_row({}, io)'.format(row)
+    out = ['const uint32_t io = (uint32_t)i * 8u;']
+    tag = 0
+    for l in lines:
+        mo = _ROW_STORE.match(l)
+        if mo:
+            # each store forms its own row address from a fresh copy of
+            # ld: k ld is not shared with the load of row k at the start
+            tag += 1
+            l = '{}crs::st_row(W + {} * crs::fresh_s<{}>(ld), io, {});'.format(
+                mo.group(1), mo.group(2), tag, mo.group(3))
+        out.append(_ROW_LOAD.sub(load, l))
+    return out
+
+
+def _launder_blocks(lines):
+    """Give every reaction block its own copies of the register-held
+    per-state values it reads (H_k, G_k, log2 T, 1/T) through crs::fresh.
+
+    Without them the compiler merges equal subexpressions of different
+    blocks (a squared H_k, a collider's G_k H_k, b log2 T for two reactions
+    with the same b) into one value that stays live from the first block to
+    the last: at 53 species that held ~50 extra FP64 values across the
+    reaction list.  The copies emit no instruction.
+    """
+    out = []
+    tag = 0
+    i = 0
+    while i < len(lines):
+        l = lines[i]
+        if not l.startswith('{  // rxn'):
+            out.append(l)
+            i += 1
+            continue
+        j = i + 1
+        while lines[j] != '}':
+            j += 1
+        body = lines[i + 1:j]
+        declared = set()
+        for b in body:
+            mo = re.match(r'\s*const (?:double|float) (\w+) =', b)
+            if mo:
+                declared.add(mo.group(1))
+        used = []
+        for b in body:
+            for mo in _FRESH_NAMES.finditer(b):
+                n = mo.group(1)
+                if n not in declared and n not in used:
+                    used.append(n)
+        tag += 1
+        out.append(l)
+        out += ['    const double {n}_f = crs::fresh<{t}>({n});'.format(
+            n=n, t=tag) for n in used]
+        ren = set(used)
+        out += [_FRESH_NAMES.sub(lambda mo: mo.group(1) + '_f'
+                                 if mo.group(1) in ren else mo.group(1), b)
+                for b in body]
+        out.append('}')
+        i = j + 1
+    return out
 
 
 def _idx_kernel_tail(fill, call, preload):
@@ -2172,10 +2324,10 @@ def _abi_text(ir, mixed=True):
     npd = ir.npd
     modes = '(1 << CRS_MODE_FP64)' + (' | (1 << CRS_MODE_MIXED)' if mixed else '')
     if mixed:
-        launch_full = '''    if (mode == CRS_MODE_MIXED)
-        k_wdot_mixed<<<crs::grid_for(n, kWdotBlock), kWdotBlock, 0, s>>>(
-            K, T, P, Y, wdot, nullptr, n, nullptr, ld);
-    else
+        launch_full = '''        if (mode == CRS_MODE_MIXED)
+            k_wdot_mixed<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
+                K, Tc, Pc, Yc, Wc, nullptr, c, nullptr, ld);
+        else
 '''
         launch_sub = '''    if (mode == CRS_MODE_MIXED)
         k_wdot_mixed<<<crs::grid_for(m, kWdotBlock), kWdotBlock, 0, s>>>(
@@ -2329,8 +2481,15 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
-{launch_full}    k_wdot_f64<<<crs::grid_for(n, kWdotBlock), kWdotBlock, 0, s>>>(
-        K, T, P, Y, wdot, nullptr, n, nullptr, ld);
+    // the fused kernels address a state with a 32-bit byte offset from its
+    // row: batches beyond kMaxChunk states go in chunks of shifted rows
+    for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
+        const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
+        const double *Tc = T + c0, *Pc = P + c0, *Yc = Y + c0;
+        double *Wc = wdot + c0;
+{launch_full_c}        k_wdot_f64<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
+            K, Tc, Pc, Yc, Wc, nullptr, c, nullptr, ld);
+    }}
     return crs::launched(fn);
 }}
 
@@ -2350,7 +2509,11 @@ int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
-    k_dydt_f64<<<crs::grid_for(n), crs::kBlock, 0, s>>>(K, P, y, dy, n, ld);
+    for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
+        const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
+        k_dydt_f64<<<crs::grid_for(c), crs::kBlock, 0, s>>>(
+            K, P + c0, y + c0, dy + c0, c, ld);
+    }}
     return crs::launched(fn);
 }}
 
@@ -2367,6 +2530,9 @@ int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
     CRS_TRY(crs::check_ptr(fn, P, "P", n));
     CRS_TRY(crs::check_ptr(fn, y, "y", n));
     CRS_TRY(crs::check_ptr(fn, work, "work", n));
+    if (n > crs::kMaxChunk)
+        return crs::fail(CRS_ERR_ARG, "%s: n = %lld exceeds %lld states per "
+                         "call", fn, (long long)n, (long long)crs::kMaxChunk);
     if ((T0 == nullptr) != (t_ign == nullptr))
         return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
     if (n == 0 || nsteps == 0) return CRS_OK;
@@ -2407,6 +2573,10 @@ static int wdot_subset(const char *fn, const double *T, const double *P,
     CRS_TRY(crs::check_ptr(fn, Y, "Y", m));
     CRS_TRY(crs::check_ptr(fn, wdot, "wdot", m));
     CRS_TRY(crs::check_ptr(fn, idx, "idx", m));
+    if (ld > crs::kMaxChunk)
+        return crs::fail(CRS_ERR_ARG, "%s: ld = %lld exceeds %lld (state "
+                         "indices are byte offsets of 32 bits)", fn,
+                         (long long)ld, (long long)crs::kMaxChunk);
     if (m == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
@@ -2437,7 +2607,7 @@ int crs_eval_wdot_subset_dev(const double *T, const double *P,
 
 }}  // extern "C"
 '''.format(name=ir.name, pm_launch=pm_launch, modes=modes,
-           launch_full=launch_full, launch_sub=launch_sub, mixed_ok=mixed_ok)
+           launch_full_c=launch_full, launch_sub=launch_sub, mixed_ok=mixed_ok)
 
 
 def write_hip(out_dir, specs, reacs, name='mech', spec_order=None, opts=None,

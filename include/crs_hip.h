@@ -115,8 +115,9 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
  * crs_eval_wdot on a subset: for t < m, state idx[t] of the [.][ld]
  * arrays is evaluated and written to column idx[t] of wdot; other columns
  * are untouched.  Used to re-evaluate gated states exactly.  Indices must
- * lie in [0, ld); idx is a device array.  No counterpart in the reference
- * (its states are all evaluated every call).
+ * lie in [0, ld); idx is a device array; ld is at most 2^29 (the kernels
+ * address a state by a 32-bit byte offset).  No counterpart in the
+ * reference (its states are all evaluated every call).
  */
 int crs_eval_wdot_subset(const double *T, const double *P, const double *Y,
                          double *wdot, const int32_t *idx, int64_t m,
@@ -167,6 +168,7 @@ int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
  * the end of a step gets t_ign = the time at the START of that step (the
  * reference sets t_ign = t at :271 before its `t += h` at :276); step k
  * starts at t0 plus k additions of h, as the reference accumulates t.
+ * n is at most 2^29 states per call.
  * Enqueues 8 kernels per step on the stream (capturable in a graph; no
  * memset, no allocation).
  */

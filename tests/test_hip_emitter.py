@@ -196,6 +196,55 @@ def test_wdot_block_option():
     text = hip_emitter.emit_hip_source(ir, {'block': 128})
     assert 'constexpr int kWdotBlock = 128;' in text
     assert 'sGl[kWdotBlock / 64]' in text
-    assert 'k_wdot_f64<<<crs::grid_for(n, kWdotBlock), kWdotBlock' in text
+    assert 'k_wdot_f64<<<crs::grid_for(c, kWdotBlock), kWdotBlock' in text
     with pytest.raises(ValueError):
         hip_emitter.emit_hip_source(ir, {'block': 100})
+
+
+def test_row_offsets_rewrite():
+    """Fused kernels address [rows][ld] arrays through a uniform row pointer
+    and one 32-bit lane offset; each late store forms its own row address."""
+    lines = ['const double T = T_[i];', 'y[3] = Y[3 * ld + i];',
+             'W[7 * ld + i] = w[7];', '    W[2 * ld + i] = 0.0;']
+    out = hip_emitter._row_offsets(lines)
+    assert out[0] == 'const uint32_t io = (uint32_t)i * 8u;'
+    assert out[1] == 'const double T = crs::ld_row(T_, io);'
+    assert out[2] == 'y[3] = crs::ld_row(Y + 3 * ld, io);'
+    assert out[3] == 'crs::st_row(W + 7 * crs::fresh_s<1>(ld), io, w[7]);'
+    assert out[4] == '    crs::st_row(W + 2 * crs::fresh_s<2>(ld), io, 0.0);'
+
+
+def test_split_blocks_groups_and_hoists():
+    """Every `per` reaction blocks form one `if (crs::go<N>())` group; a
+    value a group declares for a later group is declared before it."""
+    lines = ['const double a = 1.0;',
+             '{  // rxn 0', '    const double kf = crs::exp2_64(x);', '}',
+             'CRS_LDS double *const glp1 = crs::opaque_lds(gb);',
+             'const double gl1_4 = glp1[4 * 64];',
+             '{  // rxn 1', '    const double kf = gl0_1;', '}',
+             '{  // rxn 2', '    const double kf = gl1_4;', '}',
+             'W[0] = 0.0;']
+    out = hip_emitter._split_blocks(lines, 2)
+    assert out[0] == 'const crs::ExpK ek = crs::exp_consts();'
+    assert '    const double kf = crs::exp2_64k(ek, x);' in out
+    g1 = out.index('if (crs::go<1>()) {')
+    g2 = out.index('if (crs::go<2>()) {')
+    assert out.index('double gl1_4;') < g1 < out.index('gl1_4 = glp1[4 * 64];')
+    assert 'const double gl1_4 = glp1[4 * 64];' not in out
+    assert out.index('gl1_4 = glp1[4 * 64];') < g2
+    assert out[-1] == 'W[0] = 0.0;' and out[-2] == '}'
+
+
+def test_fused_kernels_use_the_transforms():
+    ir = _ir('mech13')
+    text = hip_emitter.emit_hip_source(ir)
+    body = text[text.index('void wdot_f64_state'):
+                text.index('void k_wdot_f64')]
+    assert 'crs::ld_row(Y + 0 * ld, io)' in body
+    assert 'if (crs::go<1>()) {' in body
+    assert 'crs::exp2_64k(ek, ' in body and 'crs::exp2_64(' not in body
+    plain = hip_emitter.emit_hip_source(ir, {'split_blocks': 0,
+                                             'row_offsets': False})
+    body = plain[plain.index('void wdot_f64_state'):
+                 plain.index('void k_wdot_f64')]
+    assert 'crs::go<' not in body and 'Y[0 * ld + i]' in body
