@@ -443,11 +443,27 @@ def main():
             T2d, P2d, Y2d, 'mixed', out=Wmx), steps, 1)
         S = gross_scale_dev(torch, lib, specs, reacs, T2d, P2d, Y2d)
         mixed = dict(workload='w2_53: {} states'.format(w2.n_states),
+                     arithmetic='FP64 Kc (G_k), net rate and sums; Arrhenius '
+                                'and falloff transcendentals at ~1e-8 '
+                                '(degree-4 exponential)',
                      states_per_s=w2.n_states / (mx_ms * 1e-3),
                      fp64_states_per_s=w2.n_states / (f64_ms * 1e-3),
                      kernel_ms=mx_k, fp64_kernel_ms=f64_k,
                      speedup_vs_fp64=f64_k / mx_k,
                      err_vs_fp64=err_summary(torch, Wmx, W64, S))
+        # the FP32 variant (emitter option mixed_arith='fp32', built beside
+        # the default library): hardware FP32 exponential cores, FP32
+        # falloff and third-body blocks
+        from create_rate_subs_amd import tune
+        vpath = tune.variant_lib(wl.mech, 'fp32')
+        if os.path.exists(vpath):
+            vlib = runtime.load(vpath)
+            v_ms, v_k = timed(torch, lambda: vlib.eval_wdot(
+                T2d, P2d, Y2d, 'mixed', out=Wmx), steps, 1)
+            mixed['fp32_variant'] = dict(
+                states_per_s=w2.n_states / (v_ms * 1e-3), kernel_ms=v_k,
+                speedup_vs_fp64=f64_k / v_k,
+                err_vs_fp64=err_summary(torch, Wmx, W64, S))
         del T2d, P2d, Y2d, W64, Wmx, S
         Te, Pe, Ye = states.equilibrium_states(specs, w2.near_eq_states,
                                                w2.state_seed + 100)

@@ -212,6 +212,60 @@ __device__ __forceinline__ double exp2_64k(const ExpK &k, double t)
     return exp_tail(k, r, n);
 }
 
+// Exponentials of the mixed-precision kernel: the same reduction as exp64 /
+// exp2_64 (in FP64: n = rint(t), r = (t - n) ln2), then e^r = 1 + r (1 +
+// r q(r)) with q of degree 4 instead of 9: relative error 1.0e-8 (exact
+// emulation in exp64_coeffs.py), five FMAs fewer.  FP64 arithmetic at
+// about 27 bits of accuracy, better than the FP32 hardware exponential
+// (about 1e-7, which a species sum with cancellation amplifies up to
+// 20-fold) and about as cheap once the conversions and the quarter issue
+// rate of v_exp_f32 are counted.
+struct ExpMK {
+    double c4, c3, c2, c1, c0, ln2, log2e;
+};
+
+__device__ __forceinline__ ExpMK exp_m_consts()
+{
+    ExpMK k;
+    k.c4 = sgpr_const(0.0013926183587713494);
+    k.c3 = vgpr_const(0.008363179052722578);
+    k.c2 = sgpr_const(0.041666554617204574);
+    k.c1 = sgpr_const(0.16666576989698775);
+    k.c0 = sgpr_const(0.5);
+    k.ln2 = sgpr_const(0.6931471805599453);
+    k.log2e = sgpr_const(1.4426950408889634);
+    return k;
+}
+
+__device__ __forceinline__ double exp2_m64k(const ExpMK &k, double t)
+{
+    const double n = __builtin_rint(t);
+    const double r = (t - n) * k.ln2;
+    double q = __builtin_fma(k.c4, r, k.c3);
+    q = __builtin_fma(q, r, k.c2);
+    q = __builtin_fma(q, r, k.c1);
+    q = __builtin_fma(q, r, k.c0);
+    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
+// e^x = 2^(x log2 e) for the mixed kernel (one more rounding, of the
+// product: |x| 2^-53 relative, negligible beside 1.0e-8 for |x| < 1e4).
+__device__ __forceinline__ double exp_m64k(const ExpMK &k, double x)
+{
+    return exp2_m64k(k, x * k.log2e);
+}
+
+__device__ __forceinline__ double exp2_m64(double t)
+{
+    return exp2_m64k(exp_m_consts(), t);
+}
+
+__device__ __forceinline__ double exp_m64(double x)
+{
+    return exp_m64k(exp_m_consts(), x);
+}
+
 // exp2_64 / exp64 with q(r) by Estrin's scheme instead of Horner's: the
 // same coefficients, 3 more multiplies, a dependency chain of 5 operations
 // instead of 10 (1.20 ulp max, as Horner, by exact emulation in

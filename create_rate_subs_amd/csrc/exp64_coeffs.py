@@ -132,9 +132,29 @@ def max_error(samples=20000, seed=1):
     return worst / 2.0 ** -53
 
 
+MIXED_DEGREE = 4
+
+
+def max_error2_mixed(samples=20000, seed=5, degree=MIXED_DEGREE):
+    """Max relative error of crs::exp2_m64k (the mixed kernel's exponential:
+    the exp2_64 sequence with q of lower degree), plain relative units."""
+    cs = coefficients(degree)
+    hi, _lo = ln2_split()
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        t = rng.uniform(-1000.0, 1000.0) if i % 2 else rng.uniform(-3.0, 3.0)
+        ex = mp.power(2, mp.mpf(t))
+        worst = max(worst, float(abs(emulate2(t, cs, hi) - ex) / ex))
+    return worst
+
+
 if __name__ == '__main__':
     print('q coefficients:', [repr(c) for c in coefficients()])
     print('ln2 split:', [repr(v) for v in ln2_split()])
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 20000
     print('exp64 max error (units of 2^-53):', max_error(n))
     print('exp2_64 max error (units of 2^-53):', max_error2(n))
+    print('mixed q coefficients:', [repr(c) for c in
+                                    coefficients(MIXED_DEGREE)])
+    print('exp2_m64 max relative error:', max_error2_mixed(n))

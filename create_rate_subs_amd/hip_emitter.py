@@ -1615,13 +1615,24 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
         lines += ['const float mf = (float)m;', 'const float Tf = (float)T;',
                   '(void)mf; (void)Tf;']
     fexp = 'crs::exp64' if opts.get('exp') == 'crs' else 'exp'
-    expfn = 'crs::exp_mixed' if mixed else fexp
+    # the mixed kernel's arithmetic (mixed_arith): 'accurate' (default)
+    # keeps Kc (the G_k), the net rate and the sums in full FP64 and takes
+    # the Arrhenius and falloff transcendentals at about 32 bits (degree-5
+    # exponential, crs::exp2_m64; falloff blocks in FP64 arithmetic);
+    # 'fp32' is the FP32 variant (hardware v_exp_f32 cores, FP32 falloff
+    # and third-body blocks), faster and about 20x less accurate
+    m32 = mixed and opts.get('mixed_arith', 'accurate') == 'fp32'
+    expfn = 'crs::exp_mixed' if m32 else fexp
     # Arrhenius exponentials in base 2 (constants pre-scaled by log2 e):
-    # exp2_mixed in the mixed kernel, exp2_64 in the FP64 one
+    # exp2_64 in the FP64 kernel, exp2_m64 or exp2_mixed in the mixed one;
     # mixed_exp='fp64' keeps the mixed kernel's Arrhenius exponentials in
-    # FP64 (to measure how much of its error they carry)
-    arr2 = ('crs::exp2_mixed' if mixed and opts.get('mixed_exp') != 'fp64'
-            else 'crs::exp2_64' if opts.get('exp') == 'crs' or mixed else None)
+    # full FP64 (to measure how much of its error they carry)
+    if mixed and opts.get('mixed_exp') == 'fp64':
+        arr2 = 'crs::exp2_64'
+    elif mixed:
+        arr2 = 'crs::exp2_mixed' if m32 else 'crs::exp2_m64'
+    else:
+        arr2 = 'crs::exp2_64' if opts.get('exp') == 'crs' else None
     tmids = []
     for x in ir.rxns:
         for tm, _lo, _hi in x.kc_groups:
@@ -1648,8 +1659,10 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
             lines += dydt_lines
     hg = opts.get('kc') == 'species_hg'
     species_kc = opts.get('kc') in ('species', 'species_hg')
-    # mixed kernel: FP32 falloff / third-body blocks unless mixed_pm='fp64'
-    mixed_pm32 = mixed and opts.get('mixed_pm', 'fp32') == 'fp32'
+    # mixed kernel: FP32 falloff / third-body blocks in the 'fp32' variant
+    # (or with mixed_pm='fp32'), FP64 ones otherwise
+    mixed_pm32 = mixed and opts.get('mixed_pm',
+                                    'fp32' if m32 else 'fp64') == 'fp32'
     nasa_lds = opts.get('nasa') == 'lds'
     if nasa_lds:
         nasa_lds = max(1, int(opts.get('nasa_prefetch', 2))) if opts.get(
@@ -1796,6 +1809,13 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                           ['W[{k} * ld + i] = w[{k}];'.format(k=k)])
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
+    if mixed and not mixed_pm32:
+        # the falloff blocks' exponentials at the mixed kernel's accuracy
+        # (the G_k, before rx_start, keep exp64: Kc stays full FP64)
+        lines = lines[:rx_start] + [
+            l.replace('crs::exp64(', 'crs::exp_m64(')
+             .replace('crs::exp2_64(', 'crs::exp2_m64(')
+            for l in lines[rx_start:]]
     if dydt:
         # energy equation, rate_subs.py:1910-1925
         lines.append('W[i] = (-1.0 / (rho * cp_avg)) * esum;')
@@ -1863,7 +1883,11 @@ def _split_blocks(lines, per):
     """
     lines = ['const crs::ExpK ek = crs::exp_consts();'] + [
         l.replace('crs::exp2_64(', 'crs::exp2_64k(ek, ')
-         .replace('crs::exp64(', 'crs::exp64k(ek, ') for l in lines]
+         .replace('crs::exp64(', 'crs::exp64k(ek, ')
+         .replace('crs::exp2_m64(', 'crs::exp2_m64k(ekm, ')
+         .replace('crs::exp_m64(', 'crs::exp_m64k(ekm, ') for l in lines]
+    if any('ekm, ' in l for l in lines):
+        lines.insert(1, 'const crs::ExpMK ekm = crs::exp_m_consts();')
     # segments: ('plain', [lines]) outside groups, ('group', [lines])
     segs = []
     cur = None

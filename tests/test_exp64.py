@@ -59,3 +59,18 @@ def test_estrin_form_shares_coefficients_and_bound():
     consts = _device_constants('double exp_q_estrin(double r)')
     assert sorted(consts) == sorted(mod.coefficients())
     assert mod.max_error2_estrin(samples=3000, seed=9) <= 1.5
+
+
+def test_mixed_exponential_coefficients_and_bound():
+    """crs::exp2_m64k (the mixed kernel's exponential) holds the degree-4
+    interpolant's coefficients and stays within 1.2e-8 relative."""
+    mod = _coeffs_module()
+    text = open(os.path.join(CSRC, 'crs_common.h')).read()
+    body = text[text.index('ExpMK exp_m_consts()'):]
+    body = body[:body.index('return k;')]
+    consts = [float(v) for v in re.findall(r'const\((-?\d+\.\d+(?:e-?\d+)?)\)',
+                                           body)]
+    hi, _lo = mod.ln2_split()
+    assert consts[:5] == mod.coefficients(mod.MIXED_DEGREE)
+    assert consts[5:] == [hi, 1.4426950408889634]
+    assert mod.max_error2_mixed(samples=3000, seed=11) <= 1.2e-8
