@@ -50,7 +50,7 @@ __device__ __forceinline__ float gelu_t(float z, float *dgelu)
 }
 
 enum Epi {
-    EPI_BIAS_GELU = 0,   // Z = acc + b (fp32, optional), A = gelu(Z) (bf16)
+    EPI_BIAS_GELU = 0,   // Z = acc + b (bf16, optional), A = gelu(Z) (bf16)
     EPI_BIAS_F32 = 1,    // out = acc + b (fp32), columns < n_valid
     EPI_DGELU_BF16 = 2,  // out = acc * gelu'(Zprev) (bf16)
     EPI_ATOMIC_F32 = 3,  // out += acc (fp32 atomics; split reduction)
@@ -67,8 +67,9 @@ __global__ __launch_bounds__(256) void k_gemm(
     const __bf16 *__restrict__ A, int64_t lda, const __bf16 *__restrict__ B,
     int64_t ldb, int64_t M, int N, int64_t R, int64_t r_chunk,
     const float *__restrict__ bias, float *__restrict__ out_f32, int64_t ldo,
-    __bf16 *__restrict__ out_bf16, float *__restrict__ zsave,
-    const float *__restrict__ zprev, int n_valid)
+    __bf16 *__restrict__ out_bf16, __bf16 *__restrict__ zsave,
+    const __bf16 *__restrict__ zprev, int n_valid,
+    __bf16 *__restrict__ out_t = nullptr, int64_t ldt = 0)
 {
     __shared__ __bf16 As[TM * LROW];
     __shared__ __bf16 Bs[TN * LROW];
@@ -156,6 +157,12 @@ __global__ __launch_bounds__(256) void k_gemm(
     for (int i = 0; i < 2; ++i) {
         for (int j = 0; j < 4; ++j) {
             const int col = n0 + j * 16 + (lane & 15);
+            // bf16 outputs also go, when asked, to the transposed array
+            // out_t[col][m] (states contiguous: the weight gradient's
+            // operand), four consecutive states in one 8-byte store, zero
+            // for the padding rows M <= m < ldt
+            __bf16 tv[4];
+            const int64_t mb = m0 + wave * 32 + i * 16 + (lane >> 4) * 4;
             for (int e = 0; e < 4; ++e) {
                 const int64_t m = m0 + wave * 32 + i * 16 + (lane >> 4) * 4 + e;
                 if (m >= M)
@@ -164,18 +171,31 @@ __global__ __launch_bounds__(256) void k_gemm(
                 if (EPI == EPI_BIAS_GELU) {
                     const float z = a + bias[col];
                     if (zsave)
-                        zsave[m * ldo + col] = z;
-                    out_bf16[m * ldo + col] = f2bf(gelu_t(z, nullptr));
+                        zsave[m * ldo + col] = f2bf(z);
+                    tv[e] = f2bf(gelu_t(z, nullptr));
+                    out_bf16[m * ldo + col] = tv[e];
                 } else if (EPI == EPI_BIAS_F32) {
                     if (col < n_valid)
                         out_f32[m * ldo + col] = a + bias[col];
                 } else if (EPI == EPI_DGELU_BF16) {
                     float d;
-                    gelu_t(zprev[m * ldo + col], &d);
-                    out_bf16[m * ldo + col] = f2bf(a * d);
+                    gelu_t(bf2f(zprev[m * ldo + col]), &d);
+                    tv[e] = f2bf(a * d);
+                    out_bf16[m * ldo + col] = tv[e];
                 } else {
                     atomicAdd(&out_f32[m * ldo + col], a);
                 }
+            }
+            if ((EPI == EPI_BIAS_GELU || EPI == EPI_DGELU_BF16) && out_t &&
+                mb < ldt) {
+                for (int e = 0; e < 4; ++e)
+                    if (mb + e >= M)
+                        tv[e] = f2bf(0.0f);
+                // ldt and mb are multiples of 4: an aligned 8-byte store
+                typedef __attribute__((__vector_size__(4 * sizeof(__bf16))))
+                    __bf16 bf16x4;
+                *(bf16x4 *)(out_t + (int64_t)col * ldt + mb) =
+                    bf16x4{tv[0], tv[1], tv[2], tv[3]};
             }
         }
     }
@@ -248,47 +268,95 @@ __global__ __launch_bounds__(256) void k_decode(
 }
 
 // MSE gradient in arcsinh space: dY = 2 (yhat - y) / (n * nv), bf16, padded
-// columns zero; loss sum accumulated into *loss.
+// columns zero; loss sum accumulated into *loss.  A block covers
+// kMseRows states: threads run along the columns (coalesced loads and
+// stores), the loss is reduced in the block and added with one atomic.
+constexpr int kMseRows = 32;
+
 __global__ __launch_bounds__(256) void k_mse_grad(
     const float *__restrict__ Yh, const float *__restrict__ Yt, int64_t n,
     int nv, int ldy, int ldt, float scale, __bf16 *__restrict__ dY, int ldd,
     float *__restrict__ loss)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.x * kMseRows;
     float part = 0.f;
-    if (t < n) {
-        for (int k = 0; k < ldd; ++k) {
-            float g = 0.f;
-            if (k < nv) {
-                const float d = Yh[t * ldy + k] - Yt[t * ldt + k];
-                part += d * d;
-                g = 2.f * d * scale;
-            }
-            dY[t * ldd + k] = f2bf(g);
+    for (int e = threadIdx.x; e < kMseRows * ldd; e += 256) {
+        const int64_t t = t0 + e / ldd;
+        const int k = e % ldd;
+        if (t >= n)
+            break;
+        float g = 0.f;
+        if (k < nv) {
+            const float d = Yh[t * ldy + k] - Yt[t * ldt + k];
+            part += d * d;
+            g = 2.f * d * scale;
         }
+        dY[t * ldd + k] = f2bf(g);
     }
-    // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1)
         part += __shfl_down(part, off, 64);
-    if ((threadIdx.x & 63) == 0 && loss)
-        atomicAdd(loss, part);
+    __shared__ float wsum[4];
+    if ((threadIdx.x & 63) == 0)
+        wsum[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0 && loss)
+        atomicAdd(loss, wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
 
-// Column sums (bias gradient): db[c] += sum_m dZ[m][c], one thread per column
-// per chunk of rows.
-__global__ __launch_bounds__(256) void k_colsum(
-    const __bf16 *__restrict__ dZ, int64_t M, int N, int64_t ld,
-    int64_t rows_per_block, float *__restrict__ db)
+// dst[c][r] = src[r][c] for r < rows, c < cols (bf16), through a padded
+// 64 x 64 LDS tile (coalesced reads and writes, no bank conflicts); columns
+// r in [rows, ld_dst) of every dst row are written as zero, so a reduction
+// over a padded length reads zeros there.
+__global__ __launch_bounds__(256) void k_transpose_bf16(
+    const __bf16 *__restrict__ src, int64_t ld_src, int64_t rows, int cols,
+    __bf16 *__restrict__ dst, int64_t ld_dst)
 {
-    const int c = threadIdx.x;
-    if (c >= N)
-        return;
-    const int64_t a = (int64_t)blockIdx.x * rows_per_block;
-    const int64_t b = (a + rows_per_block < M) ? a + rows_per_block : M;
+    __shared__ __bf16 tile[64][66];
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t r = r0 + j;
+        const int c = c0 + tx;
+        tile[j][tx] = (r < rows && c < cols) ? src[r * ld_src + c]
+                                             : f2bf(0.0f);
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {
+        const int c = c0 + j;
+        const int64_t r = r0 + tx;
+        if (c < cols && r < ld_dst)
+            dst[(int64_t)c * ld_dst + r] = tile[tx][j];
+    }
+}
+
+// Row sums of a bf16 [rows][ld] array (the bias gradient from dZ^T):
+// one block per row, 256 threads, 8-element vector loads.
+__global__ __launch_bounds__(256) void k_rowsum(
+    const __bf16 *__restrict__ a, int64_t ld, int64_t len,
+    float *__restrict__ out)
+{
+    const __bf16 *row = a + (int64_t)blockIdx.x * ld;
     float s = 0.f;
-    for (int64_t m = a; m < b; ++m)
-        s += bf2f(dZ[m * ld + c]);
-    atomicAdd(&db[c], s);
+    for (int64_t m = (int64_t)threadIdx.x * 8; m < len; m += 256 * 8) {
+        if (m + 8 <= len) {
+            const bf16x8 v = *(const bf16x8 *)(row + m);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                s += bf2f(v[e]);
+        } else {
+            for (int64_t e = m; e < len; ++e)
+                s += bf2f(row[e]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_down(s, off, 64);
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[blockIdx.x] += part[0] + part[1] + part[2] + part[3];
 }
 
 // Adam with FP32 master weights; writes the BF16 working copy.
@@ -803,11 +871,14 @@ const char *crs_mlp_last_error(void) { return crs::error_buffer(); }
 
 int crs_mlp_linear_fwd(const void *X, int64_t ldx, const void *Wb, int64_t ldw,
                        const float *bias, int64_t M, int N, int K, int act,
-                       void *A_out, float *Z_out, float *Y_out, int64_t ldo,
-                       int n_valid, void *stream)
+                       void *A_out, void *Z_out, float *Y_out, int64_t ldo,
+                       int n_valid, void *AT_out, int64_t ldat, void *stream)
 {
     const char *fn = "crs_mlp_linear_fwd";
     CRS_TRY(check_gemm(fn, M, N, K));
+    if (AT_out && (ldat < ((M + 31) / 32) * 32 || ldat % 4))
+        return crs::fail(CRS_ERR_ARG, "%s: ldat must be M rounded up to "
+                         "32 or more, a multiple of 4", fn);
     if (M == 0)
         return CRS_OK;
     if (!X || !Wb || !bias)
@@ -819,7 +890,8 @@ int crs_mlp_linear_fwd(const void *X, int64_t ldx, const void *Wb, int64_t ldw,
             return crs::fail(CRS_ERR_ARG, "%s: NULL activation output", fn);
         k_gemm<true, true, EPI_BIAS_GELU><<<grid, 256, 0, s>>>(
             (const __bf16 *)X, ldx, (const __bf16 *)Wb, ldw, M, N, K, K, bias,
-            nullptr, ldo, (__bf16 *)A_out, Z_out, nullptr, N);
+            nullptr, ldo, (__bf16 *)A_out, (__bf16 *)Z_out, nullptr, N,
+            (__bf16 *)AT_out, ldat);
     } else {
         if (!Y_out)
             return crs::fail(CRS_ERR_ARG, "%s: NULL output", fn);
@@ -832,47 +904,71 @@ int crs_mlp_linear_fwd(const void *X, int64_t ldx, const void *Wb, int64_t ldw,
 
 int crs_mlp_linear_bwd_data(const void *dZ, int64_t ldz, const void *Wb,
                             int64_t ldw, int64_t M, int N, int K,
-                            const float *Zprev, void *dZprev, int64_t ldo,
-                            void *stream)
+                            const void *Zprev, void *dZprev, int64_t ldo,
+                            void *dZprevT, int64_t ldt, void *stream)
 {
     const char *fn = "crs_mlp_linear_bwd_data";
     // dX[M][K] = dZ[M][N] W[N][K]: reduction over N; columns K
     CRS_TRY(check_gemm(fn, M, K, N));
+    if (dZprevT && (ldt < ((M + 31) / 32) * 32 || ldt % 4))
+        return crs::fail(CRS_ERR_ARG, "%s: ldt must be M rounded up to 32 "
+                         "or more, a multiple of 4", fn);
     if (M == 0)
         return CRS_OK;
     if (!dZ || !Wb || !Zprev || !dZprev)
         return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     dim3 grid((unsigned)((M + TM - 1) / TM), K / TN, 1);
-    k_gemm<true, false, EPI_DGELU_BF16><<<grid, 256, 0, (hipStream_t)stream>>>(
+    // B(k, n) = W^T[k][n]: reduction-contiguous rows of the transposed W
+    k_gemm<true, true, EPI_DGELU_BF16><<<grid, 256, 0, (hipStream_t)stream>>>(
         (const __bf16 *)dZ, ldz, (const __bf16 *)Wb, ldw, M, K, N, N, nullptr,
-        nullptr, ldo, (__bf16 *)dZprev, nullptr, Zprev, K);
+        nullptr, ldo, (__bf16 *)dZprev, nullptr, (const __bf16 *)Zprev, K,
+        (__bf16 *)dZprevT, ldt);
     return crs::launched(fn);
 }
 
-int crs_mlp_linear_bwd_weight(const void *dZ, int64_t ldz, const void *X,
+int crs_mlp_linear_bwd_weight(const void *dZT, int64_t ldz, const void *XT,
                               int64_t ldx, int64_t M, int N, int K, float *dW,
                               float *db, void *stream)
 {
     const char *fn = "crs_mlp_linear_bwd_weight";
-    // dW[N][K] = sum_m dZ[m][n] X[m][k]: rows N, columns K, reduction M
+    // dW[N][K] = sum_m dZT[n][m] XT[k][m]: rows N, columns K, reduction M
+    // along the contiguous dimension of both operands
     if (M <= 0 || N <= 0 || K <= 0 || N % TN != 0 || K % TN != 0)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
-    if (!dZ || !X || !dW || !db)
-        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     const int64_t Mr = ((M + TR - 1) / TR) * TR;
-    // split the reduction so the grid covers the GPU
-    int64_t chunk = 4096;
-    const int64_t nz = (Mr + chunk - 1) / chunk;
+    if (ldz < Mr || ldx < Mr)
+        return crs::fail(CRS_ERR_ARG, "%s: leading dimensions must cover M "
+                         "rounded up to %d (zero padded)", fn, TR);
+    if (!dZT || !XT || !dW || !db)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
+    // split the reduction so that the grid covers the GPU: 256+ blocks
+    const int64_t tiles = (int64_t)((N + TM - 1) / TM) * (K / TN);
+    int64_t nz = (256 + tiles - 1) / tiles;
+    int64_t chunk = ((Mr + nz - 1) / nz + TR - 1) / TR * TR;
+    nz = (Mr + chunk - 1) / chunk;
     dim3 grid((unsigned)((N + TM - 1) / TM), K / TN, (unsigned)nz);
     hipStream_t s = (hipStream_t)stream;
-    // rows of the output are n: A(n, m) = dZ[m * ldz + n] (reduction-major)
-    k_gemm<false, false, EPI_ATOMIC_F32><<<grid, 256, 0, s>>>(
-        (const __bf16 *)dZ, ldz, (const __bf16 *)X, ldx, N, K, M, chunk,
+    k_gemm<true, true, EPI_ATOMIC_F32><<<grid, 256, 0, s>>>(
+        (const __bf16 *)dZT, ldz, (const __bf16 *)XT, ldx, N, K, Mr, chunk,
         nullptr, dW, K, nullptr, nullptr, nullptr, K);
     CRS_TRY(crs::launched(fn));
-    const int64_t rows = 512;
-    k_colsum<<<(unsigned)((M + rows - 1) / rows), 256, 0, s>>>(
-        (const __bf16 *)dZ, M, N, ldz, rows, db);
+    k_rowsum<<<(unsigned)N, 256, 0, s>>>((const __bf16 *)dZT, ldz, M, db);
+    return crs::launched(fn);
+}
+
+int crs_mlp_transpose_bf16(const void *src, int64_t ld_src, int64_t rows,
+                           int cols, void *dst, int64_t ld_dst, void *stream)
+{
+    const char *fn = "crs_mlp_transpose_bf16";
+    if (rows < 0 || cols < 0 || ld_src < cols || ld_dst < rows)
+        return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
+    if (rows == 0 || cols == 0)
+        return CRS_OK;
+    if (!src || !dst)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
+    dim3 grid((unsigned)((ld_dst + 63) / 64), (unsigned)((cols + 63) / 64));
+    k_transpose_bf16<<<grid, 256, 0, (hipStream_t)stream>>>(
+        (const __bf16 *)src, ld_src, rows, cols, (__bf16 *)dst, ld_dst);
     return crs::launched(fn);
 }
 
@@ -929,8 +1025,9 @@ int crs_mlp_mse_grad(const float *Yh, const float *Yt, int64_t n, int nv,
     if (n <= 0 || nv <= 0 || ldd < nv)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
     const float scale = 1.0f / ((float)n * (float)nv);
-    k_mse_grad<<<crs::grid_for(n), crs::kBlock, 0, (hipStream_t)stream>>>(
-        Yh, Yt, n, nv, ldy, ldt, scale, (__bf16 *)dY, ldd, loss);
+    k_mse_grad<<<(unsigned)((n + kMseRows - 1) / kMseRows), 256, 0,
+                 (hipStream_t)stream>>>(Yh, Yt, n, nv, ldy, ldt, scale,
+                                        (__bf16 *)dY, ldd, loss);
     return crs::launched(fn);
 }
 

@@ -76,9 +76,9 @@ class MlpLib(object):
 
     SIGS = {
         'crs_mlp_linear_fwd': [_P, _I64, _P, _I64, _P, _I64, _I, _I, _I, _P,
-                               _P, _P, _I64, _I, _P],
+                               _P, _P, _I64, _I, _P, _I64, _P],
         'crs_mlp_linear_bwd_data': [_P, _I64, _P, _I64, _I64, _I, _I, _P, _P,
-                                    _I64, _P],
+                                    _I64, _P, _I64, _P],
         'crs_mlp_linear_bwd_weight': [_P, _I64, _P, _I64, _I64, _I, _I, _P,
                                       _P, _P],
         'crs_mlp_features': [_P, _P, _P, _P, _I64, _I64, _I, _P, _P, _P, _I,
@@ -89,6 +89,7 @@ class MlpLib(object):
         'crs_mlp_adam': [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I, _F,
                          _P],
         'crs_mlp_to_bf16': [_P, _P, _I64, _P],
+        'crs_mlp_transpose_bf16': [_P, _I64, _I64, _I, _P, _I64, _P],
         'crs_mlp_gate': [_P, _P, _P, _P, _I64, _I64, _I, GateParams, _P, _I,
                          _P, _P, _P, _P, _P],
         'crs_mlp_pack_tiles': [_P, _I, _I, _I, _P, _P],
@@ -292,52 +293,83 @@ class Surrogate(object):
 
     # -- network ------------------------------------------------------------
     def forward(self, X, save=False):
-        """X [M][kin] bf16 -> yhat [M][kout] fp32 (columns < nsp valid)."""
+        """X [M][kin] bf16 -> yhat [M][kout] fp32 (columns < nsp valid).
+
+        With save (training) the pre-activations Z and the hidden
+        activations are kept, the latter also transposed (states
+        contiguous, zero padded to a multiple of 32 states) for the weight
+        gradients."""
         M = X.shape[0]
+        Mr = _pad(M, 32)
         h = X
-        zs, hs = [], [X]
+        zs, hs, hts = [], [X], [None]
         nl = len(self.shapes)
         for li, (n_out, n_in) in enumerate(self.shapes):
             if li < nl - 1:
                 A = self._buf('A%d' % li, (M, n_out), torch.bfloat16)
-                Z = self._buf('Z%d' % li, (M, n_out), torch.float32) if save \
-                    else None
+                Z = AT = None
+                if save:
+                    Z = self._buf('Z%d' % li, (M, n_out), torch.bfloat16)
+                    AT = self._buf('AT%d' % li, (n_out * Mr,),
+                                   torch.bfloat16)[:n_out * Mr]
                 self.mlp('crs_mlp_linear_fwd', _p(h), h.stride(0),
                          _p(self.w(li)), n_in, _p(self.b(li)), M, n_out, n_in,
-                         1, _p(A), _p(Z), None, n_out, n_out, _s())
+                         1, _p(A), _p(Z), None, n_out, n_out, _p(AT), Mr,
+                         _s())
                 zs.append(Z)
                 hs.append(A)
+                hts.append(AT)
                 h = A
             else:
                 Yh = self._buf('Yh', (M, n_out), torch.float32)
                 self.mlp('crs_mlp_linear_fwd', _p(h), h.stride(0),
                          _p(self.w(li)), n_in, _p(self.b(li)), M, n_out, n_in,
-                         0, None, None, _p(Yh), n_out, self.nout, _s())
+                         0, None, None, _p(Yh), n_out, self.nout, None, 0,
+                         _s())
         if save:
-            self._saved = (zs, hs)
+            self._saved = (zs, hs, hts)
         return Yh
 
     def backward(self, Yh, Yt, loss=None):
-        """Gradients of the mean squared arcsinh error into self.g."""
-        zs, hs = self._saved
+        """Gradients of the mean squared arcsinh error into self.g.
+
+        The weight gradient of each layer reads dZ^T and X^T (states
+        contiguous): the forward pass and the data gradients write them
+        transposed as they go; the input features and the output layer's
+        dZ (64 wide) are transposed here."""
+        zs, hs, hts = self._saved
         M = Yh.shape[0]
+        Mr = _pad(M, 32)
         nl = len(self.shapes)
         dZ = self._buf('dZ%d' % (nl - 1), (M, self.kout), torch.bfloat16)
         self.mlp('crs_mlp_mse_grad', _p(Yh), _p(Yt), M, self.nout, self.kout,
                  Yt.stride(0), _p(dZ), self.kout, _p(loss), _s())
+        dZT = self._buf('dZT_out', (self.kout * Mr,),
+                        torch.bfloat16)[:self.kout * Mr]
+        self.mlp('crs_mlp_transpose_bf16', _p(dZ), dZ.stride(0), M, self.kout,
+                 _p(dZT), Mr, _s())
+        XT0 = self._buf('XT_in', (self.kin * Mr,),
+                        torch.bfloat16)[:self.kin * Mr]
+        self.mlp('crs_mlp_transpose_bf16', _p(hs[0]), hs[0].stride(0), M,
+                 self.kin, _p(XT0), Mr, _s())
         self.g.zero_()
         for li in range(nl - 1, -1, -1):
             n_out, n_in = self.shapes[li]
-            X = hs[li]
-            self.mlp('crs_mlp_linear_bwd_weight', _p(dZ), dZ.stride(0), _p(X),
-                     X.stride(0), M, n_out, n_in, _p(self.gw(li)),
-                     _p(self.gb(li)), _s())
+            XT = hts[li] if li > 0 else XT0
+            self.mlp('crs_mlp_linear_bwd_weight', _p(dZT), Mr, _p(XT), Mr, M,
+                     n_out, n_in, _p(self.gw(li)), _p(self.gb(li)), _s())
             if li > 0:
                 dprev = self._buf('dZ%d' % (li - 1), (M, n_in), torch.bfloat16)
+                dprevT = self._buf('dZT%d' % (li - 1), (n_in * Mr,),
+                                   torch.bfloat16)[:n_in * Mr]
+                WT = self._buf('WT%d' % li, (n_in * n_out,),
+                               torch.bfloat16)
+                self.mlp('crs_mlp_transpose_bf16', _p(self.w(li)), n_in,
+                         n_out, n_in, _p(WT), n_out, _s())
                 self.mlp('crs_mlp_linear_bwd_data', _p(dZ), dZ.stride(0),
-                         _p(self.w(li)), n_in, M, n_out, n_in, _p(zs[li - 1]),
-                         _p(dprev), n_in, _s())
-                dZ = dprev
+                         _p(WT), n_out, M, n_out, n_in, _p(zs[li - 1]),
+                         _p(dprev), n_in, _p(dprevT), Mr, _s())
+                dZ, dZT = dprev, dprevT
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
         self.step += 1
@@ -369,10 +401,45 @@ class Surrogate(object):
             self.gate.p_lo = float(P.min())
             self.gate.p_hi = float(P.max())
 
-    def train(self, T, P, Y, W, epochs=10, batch=16384, lr=2e-3,
-              lr_final=1e-4, seed=0, log=None):
-        """Train on states (T, P, Y) labelled with exact rates W [NSP][n]."""
+    def mse(self, T, P, Y, W, chunk=1 << 18):
+        """Mean squared error of the network in its standardised output
+        space over states labelled with exact rates W (the training loss)."""
         n = T.numel()
+        tot = 0.0
+        for a in range(0, n, chunk):
+            b = min(n, a + chunk)
+            Tc, Pc = T[a:b].contiguous(), P[a:b].contiguous()
+            X = self.features(Tc, Pc, Y[:, a:b].contiguous())
+            Yh = self.forward(X)[:, :self.nout]
+            Yt = self.targets(W[:, a:b].contiguous())
+            tot += ((Yh - Yt) ** 2).sum().item()
+        return tot / (n * self.nout)
+
+    def grad_step(self, X, Yt, sel, loss):
+        """Forward and backward over the batch X[sel], Yt[sel]: gradients
+        into self.g, the batch's squared error added to loss.  Reads and
+        writes only buffers of fixed address, so a captured graph of it can
+        be replayed with new indices in sel."""
+        xb = self._buf('xb', (sel.numel(), self.kin), torch.bfloat16)
+        yb = self._buf('yb', (sel.numel(), self.nout), torch.float32)
+        torch.index_select(X, 0, sel, out=xb)
+        torch.index_select(Yt, 0, sel, out=yb)
+        Yh = self.forward(xb, save=True)
+        self.backward(Yh, yb, loss)
+
+    def train(self, T, P, Y, W, epochs=10, batch=16384, lr=2e-3,
+              lr_final=1e-4, seed=0, log=None, val=None, graph=True):
+        """Train on states (T, P, Y) labelled with exact rates W [NSP][n].
+
+        Cosine learning-rate schedule from lr to lr_final.  With graph the
+        forward and backward pass of a step is captured once as a HIP graph
+        and replayed (the batch's indices are copied into its fixed index
+        buffer first; Adam runs eagerly, its step size changing every
+        step).  Returns the training MSE per epoch; with val = (T, P, Y, W)
+        the validation MSE per epoch is kept in self.val_history.
+        """
+        n = T.numel()
+        self.val_history = []
         self.fit_normalisation(T, P, Y, W)
         X = torch.empty((n, self.kin), dtype=torch.bfloat16, device='cuda')
         self.features(T, P, Y, out=X)
@@ -381,18 +448,28 @@ class Surrogate(object):
         gen.manual_seed(seed)
         loss = torch.zeros(1, dtype=torch.float32, device='cuda')
         steps_per_epoch = max(1, n // batch)
+        batch = min(batch, n)
         total = epochs * steps_per_epoch
+        sel = torch.zeros(batch, dtype=torch.int64, device='cuda')
+        g = None
+        if graph and epochs:
+            # one eager step sizes every buffer, then the capture
+            self.grad_step(X, Yt, sel, loss)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.grad_step(X, Yt, sel, loss)
         history = []
         k = 0
         for ep in range(epochs):
             perm = torch.randperm(n, device='cuda', generator=gen)
             loss.zero_()
             for s in range(steps_per_epoch):
-                sel = perm[s * batch:(s + 1) * batch]
-                xb = X.index_select(0, sel)
-                yb = Yt.index_select(0, sel)
-                Yh = self.forward(xb, save=True)
-                self.backward(Yh, yb, loss)
+                sel.copy_(perm[s * batch:(s + 1) * batch])
+                if g is not None:
+                    g.replay()
+                else:
+                    self.grad_step(X, Yt, sel, loss)
                 frac = k / max(total - 1, 1)
                 cur = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(
                     math.pi * frac))
@@ -400,8 +477,12 @@ class Surrogate(object):
                 k += 1
             mse = loss.item() / (steps_per_epoch * batch * self.nout)
             history.append(mse)
+            if val is not None:
+                self.val_history.append(self.mse(*val))
             if log:
-                log('epoch {} mse {:.4e}'.format(ep, mse))
+                log('epoch {} mse {:.4e}{}'.format(
+                    ep, mse, ' val {:.4e}'.format(self.val_history[-1])
+                    if val is not None else ''))
         return history
 
     # -- inference ----------------------------------------------------------

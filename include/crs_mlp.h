@@ -33,26 +33,38 @@ typedef struct crs_mlp_gate_params {
 
 const char *crs_mlp_last_error(void);
 
-/* Z = X W^T + b; act != 0: A_out = gelu(Z) (bf16), Z_out (fp32, optional);
- * act == 0: Y_out = Z (fp32) for columns < n_valid.  X [M][ldx], W [N][ldw],
+/* Z = X W^T + b; act != 0: A_out = gelu(Z) (bf16), Z_out (bf16, optional),
+ * and (optional) AT_out = A_out transposed, [N][ldat] with the states
+ * contiguous and zero padding (ldat >= M rounded up to 32); act == 0:
+ * Y_out = Z (fp32) for columns < n_valid.  X [M][ldx], W [N][ldw],
  * outputs [M][ldo].  N % 64 == 0, K % 32 == 0. */
 int crs_mlp_linear_fwd(const void *X, int64_t ldx, const void *W,
                        int64_t ldw, const float *bias, int64_t M, int N,
-                       int K, int act, void *A_out, float *Z_out, float *Y_out,
-                       int64_t ldo, int n_valid, void *stream);
+                       int K, int act, void *A_out, void *Z_out, float *Y_out,
+                       int64_t ldo, int n_valid, void *AT_out, int64_t ldat,
+                       void *stream);
 
-/* dZprev = (dZ W) * gelu'(Zprev): dZ [M][ldz] (N wide), W [N][ldw] (K wide),
- * Zprev and dZprev [M][ldo] (K wide). */
-int crs_mlp_linear_bwd_data(const void *dZ, int64_t ldz, const void *W,
+/* dZprev = (dZ W) * gelu'(Zprev): dZ [M][ldz] (N wide), W given
+ * transposed, WT [K][ldw] (N wide; crs_mlp_transpose_bf16 of W [N][K]),
+ * Zprev and dZprev [M][ldo] (K wide, bf16); (optional) dZprevT = dZprev
+ * transposed, [K][ldt] as AT_out of crs_mlp_linear_fwd. */
+int crs_mlp_linear_bwd_data(const void *dZ, int64_t ldz, const void *WT,
                             int64_t ldw, int64_t M, int N, int K,
-                            const float *Zprev, void *dZprev, int64_t ldo,
-                            void *stream);
+                            const void *Zprev, void *dZprev, int64_t ldo,
+                            void *dZprevT, int64_t ldt, void *stream);
 
-/* dW[N][K] += dZ^T X and db[N] += colsum(dZ) (FP32 atomics; zero them
- * first). */
-int crs_mlp_linear_bwd_weight(const void *dZ, int64_t ldz, const void *X,
+/* dW[N][K] += dZ^T X and db[N] += colsum(dZ), from the transposed operands
+ * dZT [N][ldz] and XT [K][ldx] (states contiguous, as crs_mlp_transpose_bf16
+ * writes them; ldz and ldx at least M rounded up to 32, the padding zero).
+ * FP32 atomics into dW and db: zero them first. */
+int crs_mlp_linear_bwd_weight(const void *dZT, int64_t ldz, const void *XT,
                               int64_t ldx, int64_t M, int N, int K, float *dW,
                               float *db, void *stream);
+
+/* dst[c][r] = src[r][c] for r < rows, c < cols (bf16); dst [cols][ld_dst]
+ * with its columns rows..ld_dst-1 set to zero. */
+int crs_mlp_transpose_bf16(const void *src, int64_t ld_src, int64_t rows,
+                           int cols, void *dst, int64_t ld_dst, void *stream);
 
 /* X[t][0..kp) = normalised [ln T, ln P, ln max(Y_k, 1e-30) for k < nsp]
  * of state idx[t] (or t when idx is NULL), the last species taken as

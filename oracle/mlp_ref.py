@@ -8,7 +8,9 @@ create_rate_subs_amd/csrc/mlp.hip.  It has no counterpart in the reference
 FP64 arithmetic with the GPU kernels' BF16 roundings emulated at the same
 points (inputs, weights, hidden activations and back-propagated gradients
 are BF16; products accumulate in higher precision), so the GPU's FP32
-accumulation is the only remaining difference.  GELU in its tanh form.
+accumulation is the only remaining difference.  GELU in its tanh form; the
+pre-activations kept for the backward pass are stored as BF16, as the
+training kernels store them.
 With ``emulate=False`` every rounding is switched off and the function is
 the plain FP64 loss gradient, which tests/test_mlp_ref_pins.py checks
 against torch.autograd in float64 and against central differences.
@@ -75,7 +77,7 @@ def forward_backward(X, weights, biases, Yt, nv, emulate=True):
         z = h @ weights[l].T + biases[l]
         if l < L - 1:
             a, _ = gelu(z)
-            zs.append(r32(z))
+            zs.append(rb(z))
             h = rb(a)
             hs.append(h)
         else:

@@ -96,6 +96,47 @@ def _labelled(name, n, seed):
     return Td, Pd, Yd, W
 
 
+@pytest.mark.parametrize('name', MECHS)
+def test_captured_training_step_matches_eager(name):
+    """The training step's forward and backward pass replayed from a
+    captured HIP graph, with new batch indices each time and eager work
+    between the replays: the gradients equal an eager pass over the same
+    batch up to the order of the FP32 atomic sums of the split reduction."""
+    net = _net(name, seed=31)
+    Td, Pd, Yd, W = _labelled(name, 1 << 14, 83)
+    net.fit_normalisation(Td, Pd, Yd, W)
+    X = torch.empty((Td.numel(), net.kin), dtype=torch.bfloat16,
+                    device='cuda')
+    net.features(Td, Pd, Yd, out=X)
+    Yt = net.targets(W)
+    sel = torch.zeros(4096, dtype=torch.int64, device='cuda')
+    loss = torch.zeros(1, dtype=torch.float32, device='cuda')
+    net.grad_step(X, Yt, sel, loss)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        net.grad_step(X, Yt, sel, loss)
+    gen = torch.Generator(device='cuda')
+    gen.manual_seed(5)
+    for rep in range(3):
+        idx = torch.randperm(Td.numel(), device='cuda', generator=gen)[:4096]
+        sel.copy_(idx)
+        loss.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        g_graph, l_graph = net.g.clone(), loss.item()
+        other = hip_lib(name).eval_wdot(Td, Pd, Yd)      # unrelated work
+        loss.zero_()
+        net.grad_step(X, Yt, idx, loss)
+        torch.cuda.synchronize()
+        scale = net.g.abs().max().item()
+        diff = (g_graph - net.g).abs().max().item()
+        print(rep, 'max |dg|', diff, 'of', scale)
+        assert diff <= 1e-5 * scale
+        assert abs(l_graph - loss.item()) <= 1e-5 * abs(loss.item())
+        del other
+
+
 def test_training_lowers_loss():
     net = _net(seed=11)
     Td, Pd, Yd, W = _labelled('mech13', 1 << 15, 50)
