@@ -47,7 +47,7 @@ def parse():
     ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--no-large', action='store_true',
                     help='skip the 111-species / 784-reaction workload')
-    ap.add_argument('--sur-epochs', type=int, default=30)
+    ap.add_argument('--sur-epochs', type=int, default=120)
     ap.add_argument('--sur-target', type=float, default=1e-2)
     ap.add_argument('--probe', action='store_true',
                     help='one FP64 and one mixed launch plus a plain copy of '
@@ -248,6 +248,7 @@ def surrogate_section(args, torch, lib, specs, reacs):
 
     out = dict(workload='w4_53: train {} states, test {} states'.format(n, n))
     Tt, Pt, Yt, Wt = labelled(n, w4.state_seed)
+    Tv, Pv, Yv, Wv = labelled(1 << 18, w4.state_seed + 100)
     net = Surrogate(lib, specs, reacs, seed=w4.state_seed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -258,11 +259,54 @@ def surrogate_section(args, torch, lib, specs, reacs):
     ttrain = time.perf_counter() - t0
     out['train'] = dict(states=n, epochs=args.sur_epochs, seconds=ttrain,
                         state_epochs_per_s=n * args.sur_epochs / ttrain,
-                        mse_first=hist[0], mse_last=hist[-1])
+                        mse_first=hist[0], mse_last=hist[-1],
+                        train_mse_by_epoch=hist[::max(1, len(hist) // 12)])
+    # a second member from another initialisation, trained with the
+    # validation loss per epoch: the learning curve, and the disagreement
+    # of the two as a gate signal
+    log('surrogate: second network (validation curve, ensemble signal)')
+    net2 = Surrogate(lib, specs, reacs, seed=w4.state_seed + 1000)
+    hist2 = net2.train(Tt, Pt, Yt, Wt, epochs=args.sur_epochs, batch=16384,
+                       val=(Tv, Pv, Yv, Wv))
+    step = max(1, len(hist2) // 12)
+    out['learning_curve'] = dict(
+        epochs=list(range(1, len(hist2) + 1))[::step] + [len(hist2)],
+        train_mse=hist2[::step] + [hist2[-1]],
+        val_mse=net2.val_history[::step] + [net2.val_history[-1]],
+        schedule='cosine from 2e-3 to 1e-4 over the run')
     del Tt, Pt, Yt, Wt
     # gate calibration on held-out states
-    Tv, Pv, Yv, Wv = labelled(1 << 18, w4.state_seed + 100)
     Sv = gross_scale_dev(torch, lib, specs, reacs, Tv, Pv, Yv)
+    # how well each gate signal ranks the states by error
+    Wpv = torch.empty_like(Wv)
+    net.predict(Tv, Pv, Yv, Wpv)
+    e_nw = metrics.normwise_err(Wpv, Wv)
+    e_c = metrics.rel_err(Wpv, Wv, Sv, TAU).amax(dim=0)
+    sig = dict(balance_residual=net.residuals(Wpv),
+               ensemble_disagreement=net.disagreement(net2, Tv, Pv, Yv))
+    from create_rate_subs_amd.surrogate import gate_threshold
+    gates = {}
+    for sname, sv in sig.items():
+        g = dict(spearman_vs_normwise=metrics.spearman(sv, e_nw),
+                 spearman_vs_conditioned=metrics.spearman(sv, e_c))
+        for mname, e in (('normwise', e_nw), ('conditioned', e_c)):
+            g['admitted_' + mname] = {
+                str(t): gate_threshold(e, sv, t)[1] for t in (1e-2, 1e-1)}
+        # admitting the lowest-signal fraction f of the states: the
+        # normwise error of what is admitted
+        order = torch.argsort(torch.where(torch.isfinite(sv), sv,
+                                          torch.full_like(sv, 1e300)))
+        tab = []
+        for f in (0.001, 0.01, 0.1, 0.5):
+            k = max(1, int(f * sv.numel()))
+            ea = e_nw[order[:k]].double()
+            tab.append(dict(fraction=f, normwise_max=ea.max().item(),
+                            normwise_p99=ea.quantile(0.99).item(),
+                            normwise_median=ea.median().item()))
+        g['lowest_signal_fraction'] = tab
+        gates[sname] = g
+    out['gate_signals'] = gates
+    del Wpv, net2
     _r, _a, curve_nw = net.calibrate_gate(Tv, Pv, Yv, Wv, Sv, args.sur_target,
                                           TAU, metric='normwise')
     rmax, admitted, curve = net.calibrate_gate(Tv, Pv, Yv, Wv, Sv,

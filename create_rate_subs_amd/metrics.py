@@ -121,3 +121,19 @@ def normwise_err(w_hat, w):
         out = d / m
     out = np.where(d == 0, 0.0, out)
     return np.where(np.isnan(out), np.inf, out)
+
+
+def spearman(a, b):
+    """Spearman rank correlation of two per-state vectors (torch); non-finite
+    values rank above every finite one."""
+    import torch
+
+    def rank(x):
+        x = x.double()
+        x = torch.where(torch.isfinite(x), x, torch.full_like(x, 1e300))
+        r = torch.empty_like(x)
+        r[torch.argsort(x, stable=True)] = torch.arange(
+            x.numel(), device=x.device, dtype=x.dtype)
+        return r - r.mean()
+    ra, rb = rank(a), rank(b)
+    return float((ra * rb).sum() / (ra.norm() * rb.norm()))

@@ -622,6 +622,21 @@ class Surrogate(object):
                     state_p99=per_state.quantile(0.99).item()
                     if m > 1 else per_state.max().item())
 
+    def disagreement(self, other, T, P, Y):
+        """Per-state normwise disagreement of two networks' rates,
+        max_k |w_k - w'_k| / max(max_k |w_k|, max_k |w'_k|): an error
+        estimate for gating (two members of an ensemble trained from
+        different initialisations disagree most where both are poor)."""
+        Wa = torch.empty((self.nsp, T.numel()), dtype=torch.float64,
+                         device='cuda')
+        Wb = torch.empty_like(Wa)
+        self.predict(T, P, Y, Wa)
+        other.predict(T, P, Y, Wb)
+        d = (Wa - Wb).abs().amax(dim=0)
+        m = torch.maximum(Wa.abs().amax(dim=0), Wb.abs().amax(dim=0))
+        out = d / m
+        return torch.where(d == 0, torch.zeros_like(out), out)
+
     def residuals(self, W):
         """Per-state max relative element/mass balance residual of W."""
         a = torch.cat([self.elem, self.mw[None, :]], dim=0)     # [E+1][NSP]

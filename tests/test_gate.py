@@ -52,3 +52,25 @@ def test_admitted_states_meet_the_target(seed):
         passed = r <= rmax if frac > 0 else torch.zeros(n, dtype=torch.bool)
         assert passed.double().mean().item() == pytest.approx(frac)
         assert (e[passed] <= tgt).all()
+
+
+def test_spearman_rank_correlation():
+    import torch
+    from create_rate_subs_amd import metrics
+    a = torch.tensor([0.1, 3.0, 2.0, 5.0, 4.0], dtype=torch.float64)
+    assert abs(metrics.spearman(a, a ** 3) - 1.0) < 1e-12
+    assert abs(metrics.spearman(a, -a) + 1.0) < 1e-12
+    # against scipy's definition on random data without ties
+    import numpy as np
+    from scipy.stats import spearmanr
+    rng = np.random.default_rng(3)
+    x, y = rng.normal(size=500), rng.normal(size=500)
+    y = y + 0.5 * x
+    ref = spearmanr(x, y).correlation
+    got = metrics.spearman(torch.from_numpy(x), torch.from_numpy(y))
+    assert abs(got - ref) < 1e-12
+    # non-finite values rank last
+    b = torch.tensor([1.0, float('inf'), 2.0, float('nan')],
+                     dtype=torch.float64)
+    c = torch.tensor([1.0, 4.0, 2.0, 3.0], dtype=torch.float64)
+    assert metrics.spearman(b, c) > 0.7
