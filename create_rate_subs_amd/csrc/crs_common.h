@@ -23,6 +23,12 @@ constexpr int kBlock = 256;
 // within a row, 8 x index, is a 32-bit value (ld_row / st_row below).
 constexpr int64_t kMaxChunk = (int64_t)1 << 29;
 
+// The host helpers below have internal linkage: every library that
+// includes this file (each mechanism library, the surrogate library) keeps
+// its own error text, so one library's failure never overwrites the
+// message another library's crs_last_error / crs_mlp_last_error reports.
+namespace {
+
 inline char *error_buffer()
 {
     static thread_local char buf[512] = {0};
@@ -73,6 +79,8 @@ inline dim3 grid_for(int64_t n, int block = kBlock)
 {
     return dim3((unsigned)((n + block - 1) / block));
 }
+
+}  // namespace
 
 // 1/x to within an ulp: hardware reciprocal estimate refined by two
 // Newton-Raphson steps (each doubles the correct bits).  For positive

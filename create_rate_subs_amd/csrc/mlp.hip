@@ -978,7 +978,8 @@ int crs_mlp_features(const double *T, const double *P, const double *Y,
                      void *stream)
 {
     const char *fn = "crs_mlp_features";
-    if (n < 0 || ld < 0 || nsp < 2 || kp < nsp + 2)
+    // without idx, states 0..n-1 are read: ld must cover them
+    if (n < 0 || ld < 0 || (!idx && ld < n) || nsp < 2 || kp < nsp + 2)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
     if (n == 0)
         return CRS_OK;
@@ -994,10 +995,12 @@ int crs_mlp_targets(const double *W, int64_t n, int64_t ld, int nsp,
                     float *Yt, int ldt, void *stream)
 {
     const char *fn = "crs_mlp_targets";
-    if (n < 0 || ldt < nsp)
+    if (n < 0 || ld < n || nsp < 1 || ldt < nsp)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
     if (n == 0)
         return CRS_OK;
+    if (!W || !scale || !mu || !sd || !Yt)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     k_targets<<<crs::grid_for(n), crs::kBlock, 0, (hipStream_t)stream>>>(
         W, n, ld, nsp, scale, mu, sd, Yt, ldt);
     return crs::launched(fn);
@@ -1008,10 +1011,12 @@ int crs_mlp_decode(const float *Yh, int ldy, const int32_t *idx, int64_t n,
                    const float *sd, double *W, void *stream)
 {
     const char *fn = "crs_mlp_decode";
-    if (n < 0 || ldy < nsp)
+    if (n < 0 || ld < 0 || (!idx && ld < n) || nsp < 1 || ldy < nsp)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
     if (n == 0)
         return CRS_OK;
+    if (!Yh || !scale || !mu || !sd || !W)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     k_decode<<<crs::grid_for(n), crs::kBlock, 0, (hipStream_t)stream>>>(
         Yh, ldy, idx, n, ld, nsp, scale, mu, sd, W);
     return crs::launched(fn);
@@ -1022,8 +1027,10 @@ int crs_mlp_mse_grad(const float *Yh, const float *Yt, int64_t n, int nv,
                      void *stream)
 {
     const char *fn = "crs_mlp_mse_grad";
-    if (n <= 0 || nv <= 0 || ldd < nv)
+    if (n <= 0 || nv <= 0 || ldy < nv || ldt < nv || ldd < nv)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
+    if (!Yh || !Yt || !dY)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     const float scale = 1.0f / ((float)n * (float)nv);
     k_mse_grad<<<(unsigned)((n + kMseRows - 1) / kMseRows), 256, 0,
                  (hipStream_t)stream>>>(Yh, Yt, n, nv, ldy, ldt, scale,
@@ -1038,6 +1045,8 @@ int crs_mlp_adam(float *p, const float *g, float *m1, float *m2, void *pb,
     const char *fn = "crs_mlp_adam";
     if (n <= 0 || step <= 0)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
+    if (!p || !g || !m1 || !m2 || !pb)
+        return crs::fail(CRS_ERR_ARG, "%s: NULL pointer", fn);
     const float c1 = 1.f - powf(b1, (float)step);
     const float c2 = 1.f - powf(b2, (float)step);
     k_adam<<<crs::grid_for(n), crs::kBlock, 0, (hipStream_t)stream>>>(
@@ -1047,8 +1056,12 @@ int crs_mlp_adam(float *p, const float *g, float *m1, float *m2, void *pb,
 
 int crs_mlp_to_bf16(const float *a, void *b, int64_t n, void *stream)
 {
-    if (n <= 0)
+    if (n < 0)
+        return crs::fail(CRS_ERR_ARG, "crs_mlp_to_bf16: n must be >= 0");
+    if (n == 0)
         return CRS_OK;
+    if (!a || !b)
+        return crs::fail(CRS_ERR_ARG, "crs_mlp_to_bf16: NULL pointer");
     k_to_bf16<<<crs::grid_for(n), crs::kBlock, 0, (hipStream_t)stream>>>(
         a, (__bf16 *)b, n);
     return crs::launched("crs_mlp_to_bf16");
@@ -1061,7 +1074,7 @@ int crs_mlp_gate(const double *T, const double *P, const double *Y,
                  int32_t *fb_idx, int32_t *fb_count, void *stream)
 {
     const char *fn = "crs_mlp_gate";
-    if (n < 0 || nsp < 2 || n_elem < 0)
+    if (n < 0 || ld < n || nsp < 2 || n_elem < 0)
         return crs::fail(CRS_ERR_ARG, "%s: bad sizes", fn);
     if (n == 0)
         return CRS_OK;
