@@ -94,54 +94,53 @@ __device__ __forceinline__ double rcp64(double x)
     return __builtin_fma(r, e, r);
 }
 
-// e^x in FP64 to about 1.2 ulp (max relative error 1.19 * 2^-53 measured by
+// e^x in FP64 to about 6 ulp (max relative error 6.3 * 2^-53 measured by
 // exact emulation of the FMA sequence).  Cody-Waite reduction
 // x = n ln2 + r, |r| <= ln2/2, with ln2 split into a double and its
-// remainder; e^r = 1 + r (1 + r q(r)) where q(r) = (e^r - 1 - r) / r^2 is
-// the degree-9 Chebyshev interpolant on [-ln2/2, ln2/2] (coefficients from
-// exp64_coeffs.py beside this file); the result is scaled by 2^n with
-// ldexp, which also delivers overflow to +inf and underflow to 0.  It takes
-// no branch and no range test, unlike the library exp: finite arguments and
-// NaN only (exp(-inf) would give NaN, which no rate expression produces).
+// remainder; e^r = 1 + r Q(r) where Q(r) = (e^r - 1) / r is the degree-9
+// Chebyshev interpolant on [-ln2/2, ln2/2] (coefficients from
+// exp64_coeffs.py beside this file: ten coefficients, nine Horner FMAs and
+// one final FMA); the result is scaled by 2^n with ldexp, which also
+// delivers overflow to +inf and underflow to 0.  It takes no branch and no
+// range test, unlike the library exp: finite arguments and NaN only
+// (exp(-inf) would give NaN, which no rate expression produces).
 __device__ __forceinline__ double exp64(double x)
 {
     const double n = __builtin_rint(x * 1.4426950408889634);
     double r = __builtin_fma(-n, 0.6931471805599453, x);
     r = __builtin_fma(-n, 2.3190468138462996e-17, r);
-    double q = 2.510038549551032e-08;
-    q = __builtin_fma(q, r, 2.7620088445409746e-07);
-    q = __builtin_fma(q, r, 2.7557268459997064e-06);
+    double q = 2.7620088445409746e-07;
+    q = __builtin_fma(q, r, 2.763265575933203e-06);
     q = __builtin_fma(q, r, 2.4801521295954376e-05);
-    q = __builtin_fma(q, r, 0.00019841269863053618);
+    q = __builtin_fma(q, r, 0.0001984119061583585);
     q = __builtin_fma(q, r, 0.0013888888917213717);
-    q = __builtin_fma(q, r, 0.008333333333330062);
+    q = __builtin_fma(q, r, 0.008333333367332009);
     q = __builtin_fma(q, r, 0.04166666666662413);
-    q = __builtin_fma(q, r, 0.16666666666666669);
+    q = __builtin_fma(q, r, 0.16666666666615607);
     q = __builtin_fma(q, r, 0.5000000000000001);
-    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
-    return __builtin_ldexp(p, (int)n);
+    q = __builtin_fma(q, r, 1.0000000000000013);
+    return __builtin_ldexp(__builtin_fma(r, q, 1.0), (int)n);
 }
 
 // 2^t in FP64, same polynomial: t = n + s with n = rint(t) and s = t - n
-// exact, r = s ln2 (one rounding, |r| <= ln2/2), then as exp64.  For
-// exponents formed in base 2 (constants pre-scaled by log2 e), which saves
-// exp64's conversion multiply and its two-step reduction.
+// exact, r = s ln2 (one rounding, |r| <= ln2/2), then as exp64 (6.2 ulp
+// max).  For exponents formed in base 2 (constants pre-scaled by log2 e),
+// which saves exp64's conversion multiply and its two-step reduction.
 __device__ __forceinline__ double exp2_64(double t)
 {
     const double n = __builtin_rint(t);
     const double r = (t - n) * 0.6931471805599453;
-    double q = 2.510038549551032e-08;
-    q = __builtin_fma(q, r, 2.7620088445409746e-07);
-    q = __builtin_fma(q, r, 2.7557268459997064e-06);
+    double q = 2.7620088445409746e-07;
+    q = __builtin_fma(q, r, 2.763265575933203e-06);
     q = __builtin_fma(q, r, 2.4801521295954376e-05);
-    q = __builtin_fma(q, r, 0.00019841269863053618);
+    q = __builtin_fma(q, r, 0.0001984119061583585);
     q = __builtin_fma(q, r, 0.0013888888917213717);
-    q = __builtin_fma(q, r, 0.008333333333330062);
+    q = __builtin_fma(q, r, 0.008333333367332009);
     q = __builtin_fma(q, r, 0.04166666666662413);
-    q = __builtin_fma(q, r, 0.16666666666666669);
+    q = __builtin_fma(q, r, 0.16666666666615607);
     q = __builtin_fma(q, r, 0.5000000000000001);
-    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
-    return __builtin_ldexp(p, (int)n);
+    q = __builtin_fma(q, r, 1.0000000000000013);
+    return __builtin_ldexp(__builtin_fma(r, q, 1.0), (int)n);
 }
 
 // The constants of exp64 / exp2_64 as run-time values held in registers.
@@ -151,9 +150,9 @@ __device__ __forceinline__ double exp2_64(double t)
 // them again (two scalar moves per constant, plus a vector move for the one
 // that must sit in a VGPR: a VALU instruction takes one scalar operand).
 // The values pass through empty asm so the compiler cannot fold them back
-// into literals; c8 is kept in a VGPR pair for the first Horner step.
+// into literals; e8 is kept in a VGPR pair for the first Horner step.
 struct ExpK {
-    double c9, c8, c7, c6, c5, c4, c3, c2, c1, c0;
+    double e9, e8, e7, e6, e5, e4, e3, e2, e1, e0;
     double ln2, ln2lo, log2e;
 };
 
@@ -172,16 +171,16 @@ __device__ __forceinline__ double vgpr_const(double x)
 __device__ __forceinline__ ExpK exp_consts()
 {
     ExpK k;
-    k.c9 = sgpr_const(2.510038549551032e-08);
-    k.c8 = vgpr_const(2.7620088445409746e-07);
-    k.c7 = sgpr_const(2.7557268459997064e-06);
-    k.c6 = sgpr_const(2.4801521295954376e-05);
-    k.c5 = sgpr_const(0.00019841269863053618);
-    k.c4 = sgpr_const(0.0013888888917213717);
-    k.c3 = sgpr_const(0.008333333333330062);
-    k.c2 = sgpr_const(0.04166666666662413);
-    k.c1 = sgpr_const(0.16666666666666669);
-    k.c0 = sgpr_const(0.5000000000000001);
+    k.e9 = sgpr_const(2.7620088445409746e-07);
+    k.e8 = vgpr_const(2.763265575933203e-06);
+    k.e7 = sgpr_const(2.4801521295954376e-05);
+    k.e6 = sgpr_const(0.0001984119061583585);
+    k.e5 = sgpr_const(0.0013888888917213717);
+    k.e4 = sgpr_const(0.008333333367332009);
+    k.e3 = sgpr_const(0.04166666666662413);
+    k.e2 = sgpr_const(0.16666666666615607);
+    k.e1 = sgpr_const(0.5000000000000001);
+    k.e0 = sgpr_const(1.0000000000000013);
     k.ln2 = sgpr_const(0.6931471805599453);
     k.ln2lo = sgpr_const(2.3190468138462996e-17);
     k.log2e = sgpr_const(1.4426950408889634);
@@ -192,17 +191,16 @@ __device__ __forceinline__ ExpK exp_consts()
 // the same order, so the same results bit for bit.
 __device__ __forceinline__ double exp_tail(const ExpK &k, double r, double n)
 {
-    double q = __builtin_fma(k.c9, r, k.c8);
-    q = __builtin_fma(q, r, k.c7);
-    q = __builtin_fma(q, r, k.c6);
-    q = __builtin_fma(q, r, k.c5);
-    q = __builtin_fma(q, r, k.c4);
-    q = __builtin_fma(q, r, k.c3);
-    q = __builtin_fma(q, r, k.c2);
-    q = __builtin_fma(q, r, k.c1);
-    q = __builtin_fma(q, r, k.c0);
-    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
-    return __builtin_ldexp(p, (int)n);
+    double q = __builtin_fma(k.e9, r, k.e8);
+    q = __builtin_fma(q, r, k.e7);
+    q = __builtin_fma(q, r, k.e6);
+    q = __builtin_fma(q, r, k.e5);
+    q = __builtin_fma(q, r, k.e4);
+    q = __builtin_fma(q, r, k.e3);
+    q = __builtin_fma(q, r, k.e2);
+    q = __builtin_fma(q, r, k.e1);
+    q = __builtin_fma(q, r, k.e0);
+    return __builtin_ldexp(__builtin_fma(r, q, 1.0), (int)n);
 }
 
 __device__ __forceinline__ double exp64k(const ExpK &k, double x)
@@ -274,10 +272,11 @@ __device__ __forceinline__ double exp_m64(double x)
     return exp_m64k(exp_m_consts(), x);
 }
 
-// exp2_64 / exp64 with q(r) by Estrin's scheme instead of Horner's: the
-// same coefficients, 3 more multiplies, a dependency chain of 5 operations
-// instead of 10 (1.20 ulp max, as Horner, by exact emulation in
-// exp64_coeffs.py).
+// exp2_64 / exp64 in the q form, e^r = 1 + r (1 + r q(r)) with
+// q(r) = (e^r - 1 - r) / r^2 of degree 9 (exp64_coeffs.py coefficients()),
+// q by Estrin's scheme: 3 more multiplies than Horner's, a dependency
+// chain of 5 operations instead of 10 (1.20 ulp max by exact emulation in
+// exp64_coeffs.py).  An emitter option (exp_poly), measured slower.
 __device__ __forceinline__ double exp_q_estrin(double r)
 {
     const double r2 = r * r;

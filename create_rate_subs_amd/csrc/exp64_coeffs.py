@@ -1,10 +1,15 @@
-"""Coefficients of crs::exp64 (crs_common.h) and their accuracy check.
+"""Coefficients of crs::exp64 / crs::exp2_64 (crs_common.h) and their
+accuracy check.
 
-e^r = 1 + r (1 + r q(r)) on |r| <= ln2/2, with q(r) = (e^r - 1 - r) / r^2
-approximated by its degree-9 Chebyshev interpolant (mpmath.chebyfit, 60
-digits).  The check emulates the device sequence exactly: every FMA is
-rounded once to double, the reduction uses ln2 split into a double and its
-remainder.  Run: python exp64_coeffs.py [samples]
+e^r = 1 + r Q(r) on |r| <= ln2/2, with Q(r) = (e^r - 1) / r approximated by
+its degree-9 Chebyshev interpolant (mpmath.chebyfit, 60 digits): ten
+coefficients, nine Horner FMAs and one final FMA.  The earlier form,
+e^r = 1 + r (1 + r q(r)) with q(r) = (e^r - 1 - r) / r^2 of degree 9
+(``coefficients``), is more accurate (1.2 ulp against 6.3) at one FMA more
+per exponential; it remains in crs::exp_q_estrin.  The checks emulate the
+device sequences exactly: every FMA is rounded once to double, the
+reduction uses ln2 split into a double and its remainder.
+Run: python exp64_coeffs.py [samples]
 """
 
 import random
@@ -28,6 +33,83 @@ def coefficients(degree=9):
     poly, _err = mp.chebyfit(q, [-a * 1.0001, a * 1.0001], degree + 1,
                              error=True)
     return [float(c) for c in poly]
+
+
+def coefficients_e(degree=9):
+    """Q's coefficients (Q(r) = (e^r - 1) / r), highest power first,
+    rounded to double: those of crs::exp64 and crs::exp2_64."""
+    mp.mp.dps = 60
+    a = mp.log(2) / 2
+
+    def Q(r):
+        if abs(r) < mp.mpf('1e-25'):
+            return mp.mpf(1)
+        return (mp.exp(r) - 1) / r
+
+    poly, _err = mp.chebyfit(Q, [-a * 1.0001, a * 1.0001], degree + 1,
+                             error=True)
+    return [float(c) for c in poly]
+
+
+def _tail_e(r, cs, fma):
+    q = mp.mpf(cs[0])
+    for c in cs[1:]:
+        q = fma(q, r, c)
+    return fma(r, q, 1.0)
+
+
+def emulate_e(x, cs, hi, lo):
+    """crs::exp64: Cody-Waite reduction, then 1 + r Q(r), each operation
+    rounded to double."""
+    def rd(v):
+        return mp.mpf(float(v))
+
+    def fma(a, b, c):
+        return rd(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+    n = mp.nint(rd(mp.mpf(x) * LOG2E))
+    r = fma(-n, hi, x)
+    r = fma(-n, lo, r)
+    return rd(mp.ldexp(_tail_e(r, cs, fma), int(n)))
+
+
+def emulate2_e(t, cs, hi):
+    """crs::exp2_64: n = rint(t), r = (t - n) ln2, then 1 + r Q(r)."""
+    def rd(v):
+        return mp.mpf(float(v))
+
+    def fma(a, b, c):
+        return rd(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+    n = mp.nint(t)
+    r = rd((mp.mpf(t) - n) * hi)
+    return rd(mp.ldexp(_tail_e(r, cs, fma), int(n)))
+
+
+def max_error_e(samples=20000, seed=1):
+    """Max relative error of crs::exp64 against e^x, units of 2^-53."""
+    cs = coefficients_e()
+    hi, lo = ln2_split()
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        x = rng.uniform(-700.0, 700.0) if i % 2 else rng.uniform(-2.0, 2.0)
+        ex = mp.exp(mp.mpf(x))
+        worst = max(worst, float(abs(emulate_e(x, cs, hi, lo) - ex) / ex))
+    return worst / 2.0 ** -53
+
+
+def max_error2_e(samples=20000, seed=2):
+    """Max relative error of crs::exp2_64 against 2^t, units of 2^-53."""
+    cs = coefficients_e()
+    hi, _lo = ln2_split()
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        t = rng.uniform(-1000.0, 1000.0) if i % 2 else rng.uniform(-3.0, 3.0)
+        ex = mp.power(2, mp.mpf(t))
+        worst = max(worst, float(abs(emulate2_e(t, cs, hi) - ex) / ex))
+    return worst / 2.0 ** -53
 
 
 def ln2_split():
@@ -107,7 +189,8 @@ def max_error2_estrin(samples=20000, seed=4):
 
 
 def max_error2(samples=20000, seed=2):
-    """Max relative error of crs::exp2_64 against 2^t, units of 2^-53."""
+    """Max relative error of the q-form 2^t (1 + r (1 + r q(r))) against
+    2^t, units of 2^-53."""
     cs = coefficients()
     hi, _lo = ln2_split()
     rng = random.Random(seed)
@@ -120,7 +203,8 @@ def max_error2(samples=20000, seed=2):
 
 
 def max_error(samples=20000, seed=1):
-    """Max relative error of the emulated sequence, in units of 2^-53."""
+    """Max relative error of the q-form e^x (1 + r (1 + r q(r))), units of
+    2^-53."""
     cs = coefficients()
     hi, lo = ln2_split()
     rng = random.Random(seed)
@@ -150,11 +234,15 @@ def max_error2_mixed(samples=20000, seed=5, degree=MIXED_DEGREE):
 
 
 if __name__ == '__main__':
-    print('q coefficients:', [repr(c) for c in coefficients()])
+    print('Q coefficients (exp64, exp2_64):',
+          [repr(c) for c in coefficients_e()])
+    print('q coefficients (exp_q_estrin):', [repr(c) for c in coefficients()])
     print('ln2 split:', [repr(v) for v in ln2_split()])
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 20000
-    print('exp64 max error (units of 2^-53):', max_error(n))
-    print('exp2_64 max error (units of 2^-53):', max_error2(n))
+    print('exp64 max error (units of 2^-53):', max_error_e(n))
+    print('exp2_64 max error (units of 2^-53):', max_error2_e(n))
+    print('q-form exp / exp2 max error (units of 2^-53):', max_error(n),
+          max_error2(n))
     print('mixed q coefficients:', [repr(c) for c in
                                     coefficients(MIXED_DEGREE)])
     print('exp2_m64 max relative error:', max_error2_mixed(n))

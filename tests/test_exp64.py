@@ -1,9 +1,12 @@
 """crs::exp64 (the branch-free FP64 exponential of the fused kernels).
 
 The coefficients in crs_common.h must be the ones exp64_coeffs.py derives
-(Chebyshev interpolant of (e^r - 1 - r)/r^2), and the device sequence,
-emulated with one rounding per FMA, must stay within 1.5 units of 2^-53
-relative of e^x (mpmath, 60 digits) over |x| <= 700.
+(Chebyshev interpolant of Q(r) = (e^r - 1)/r for exp64 / exp2_64 and the
+fused kernels' register-held ExpK; of q(r) = (e^r - 1 - r)/r^2 for the
+Estrin form), and the device sequences, emulated with one rounding per FMA,
+must stay within 7 units of 2^-53 relative of e^x and 2^t (mpmath, 60
+digits) over |x| <= 700, |t| <= 1000 (1.5 units for the q form).  A
+planted mistake (a dropped or perturbed coefficient) breaks the bound.
 """
 
 import importlib.util
@@ -35,12 +38,27 @@ def test_header_holds_derived_coefficients():
     hi, lo = mod.ln2_split()
     assert consts[0] == 1.4426950408889634
     assert consts[1:3] == [hi, lo]
-    assert consts[3:13] == mod.coefficients()
+    assert consts[3:13] == mod.coefficients_e()
+    assert consts[13:] == [1.0]
+
+
+def test_register_constants_match_the_literal_form():
+    """ExpK (the fused kernels' register-held constants) holds the same
+    coefficients in the same order as exp64 / exp2_64."""
+    mod = _coeffs_module()
+    text = open(os.path.join(CSRC, 'crs_common.h')).read()
+    body = text[text.index('ExpK exp_consts()'):]
+    body = body[:body.index('return k;')]
+    consts = [float(v) for v in re.findall(r'const\((-?\d+\.\d+(?:e-?\d+)?)\)',
+                                           body)]
+    hi, lo = mod.ln2_split()
+    assert consts[:10] == mod.coefficients_e()
+    assert consts[10:] == [hi, lo, 1.4426950408889634]
 
 
 def test_emulated_sequence_error_bound():
     mod = _coeffs_module()
-    assert mod.max_error(samples=3000, seed=7) <= 1.5
+    assert mod.max_error_e(samples=3000, seed=7) <= 7.0
 
 
 def test_exp2_64_shares_the_polynomial_and_error_bound():
@@ -48,13 +66,30 @@ def test_exp2_64_shares_the_polynomial_and_error_bound():
     consts = _device_constants('double exp2_64(double t)')
     hi, _lo = mod.ln2_split()
     assert consts[0] == hi
-    assert consts[1:11] == mod.coefficients()
-    assert mod.max_error2(samples=3000, seed=8) <= 1.5
+    assert consts[1:11] == mod.coefficients_e()
+    assert mod.max_error2_e(samples=3000, seed=8) <= 7.0
+
+
+def test_planted_coefficient_mistakes_break_the_bound():
+    """The bound is tight enough to see a wrong coefficient: dropping the
+    top one, or perturbing the linear one by 1e-12, exceeds 7 ulp."""
+    import mpmath as mp
+    mod = _coeffs_module()
+    hi, lo = mod.ln2_split()
+    good = mod.coefficients_e()
+    for cs in ([0.0] + good[1:], good[:-2] + [good[-2] * (1 + 1e-12),
+                                              good[-1]]):
+        worst = 0.0
+        for x in (-0.34, -0.2, 0.17, 0.3465, 5.3, -12.9):
+            ex = mp.exp(mp.mpf(x))
+            worst = max(worst, float(abs(mod.emulate_e(x, cs, hi, lo) - ex) /
+                                     ex))
+        assert worst / 2.0 ** -53 > 7.0
 
 
 def test_estrin_form_shares_coefficients_and_bound():
-    """exp_q_estrin holds the same ten coefficients (paired low/high power)
-    and the Estrin evaluation keeps the 1.5 ulp bound."""
+    """exp_q_estrin holds the q form's ten coefficients (paired low/high
+    power) and the Estrin evaluation keeps the 1.5 ulp bound."""
     mod = _coeffs_module()
     consts = _device_constants('double exp_q_estrin(double r)')
     assert sorted(consts) == sorted(mod.coefficients())
