@@ -449,7 +449,7 @@ __global__ __launch_bounds__(256) void k_gate(
 // Fused inference: features -> 3 x (256, GELU) -> 64 outputs -> decode -> gate
 // in one kernel, activations never leaving registers.
 //
-// A wave owns 64 states as four 16-state column blocks.  Every product is
+// A wave owns 32 states as two 16-state column blocks.  Every product is
 // D = W_tile (16 x 32, from LDS) x act (32 x 16 states) on
 // v_mfma_f32_16x16x32_bf16, so a layer's output lands with lane l holding
 // units 16r + 4(l/16) + t (t < 4) of state l % 16.  Two consecutive row
@@ -460,8 +460,11 @@ __global__ __launch_bounds__(256) void k_gate(
 // in that tile order (64 lanes x 8 bf16 per 16 x 32 tile), so activations
 // pass from layer to layer with no shuffle and no LDS round trip.
 //
-// The block (4 waves, 256 states) stages one layer's tiles at a time in
-// 128 KB of LDS; each tile read feeds four MFMAs (one per column block).
+// The block (4 waves, 128 states) stages the tiles of the first and output
+// layers whole and those of each 256 x 256 layer in two halves of 8 row
+// blocks, 64 KB of LDS at a time; each tile read feeds two MFMAs (one per
+// column block).  Two blocks share a CU and run apart, so one block's
+// staging overlaps the other's products and epilogues.
 // Features go through LDS once (per-lane state -> B layout), outputs once
 // (D layout -> per-lane state), then each lane decodes its state's rates to
 // FP64, applies the gate and appends fallback states with one atomic per
@@ -471,19 +474,25 @@ constexpr int IH = 256;           // hidden width
 constexpr int IKIN = 64;          // padded input width
 constexpr int IKOUT = 64;         // padded output width
 constexpr int INC = 2;            // 16-state column blocks per wave
-constexpr int IWAVES = 8;         // two per SIMD: one multiplies while
-                                  // the other runs its GELU epilogue
+constexpr int IWAVES = 4;         // 128 states per block; two blocks per
+                                  // CU (two waves per SIMD: one multiplies
+                                  // while the other runs its GELU epilogue)
+                                  // run apart, so one block's tile staging
+                                  // overlaps the other block's products
 constexpr int ITHREADS = 64 * IWAVES;
 constexpr int ISTATES = 16 * INC * IWAVES;
 constexpr int IFEAT_LD = 72;      // bf16 per state row (144 B, 16 B aligned)
 constexpr int IOUT_LD = 68;       // floats per state row
-constexpr int ILDS = 128 * 1024;  // one hidden layer of tiles
+constexpr int ILDS = 64 * 1024;   // half a hidden layer of tiles
+constexpr int IPIECES = 2;        // a 256 x 256 layer is staged in halves
 
 static_assert(2 * ISTATES == ITHREADS, "two threads per state in the state stages");
 static_assert(ISTATES * IFEAT_LD * 2 + ITHREADS * 12 <= ILDS,
               "feature stage and its exchange fit");
 static_assert(ISTATES * IOUT_LD * 4 <= ILDS, "output stage fits");
-static_assert(IH * IH * 2 <= ILDS, "hidden layer fits");
+static_assert(IH * IH * 2 / IPIECES <= ILDS, "half a hidden layer fits");
+static_assert(IH * IKIN * 2 <= ILDS && IKOUT * IH * 2 <= ILDS,
+              "first and output layers fit whole");
 
 // Tile element (lane l = i + 16 g, element e) of row block r, step s.
 __device__ __forceinline__ int tile_k(int perm, int s, int g, int e)
@@ -550,7 +559,7 @@ __device__ __forceinline__ unsigned gelu_pair(float z0, float z1)
     return w;
 }
 
-template <int NS, int NRB, bool HIDDEN>
+template <int NS, int NRB, bool HIDDEN, int RB0 = 0>
 __device__ __forceinline__ void infer_layer(const bf16x8 *wt,
                                             const float *__restrict__ bias,
                                             const u32x4 (&in)[INC][8],
@@ -569,7 +578,7 @@ __device__ __forceinline__ void infer_layer(const bf16x8 *wt,
             for (int s = 0; s < NS; ++s)
                 a_nxt[s] = wt[((r + 1) * NS + s) * 64 + lane];
         }
-        const floatx4 bv = *(const floatx4 *)(bias + 16 * r + 4 * g);
+        const floatx4 bv = *(const floatx4 *)(bias + 16 * (RB0 + r) + 4 * g);
         floatx4 acc[INC];
 #pragma unroll
         for (int b = 0; b < INC; ++b)
@@ -584,11 +593,14 @@ __device__ __forceinline__ void infer_layer(const bf16x8 *wt,
         }
 #pragma unroll
         for (int b = 0; b < INC; ++b) {
+            // row block RB0 + r of the layer
             if (HIDDEN) {
-                nxt[b][r >> 1][(r & 1) * 2] = gelu_pair(acc[b][0], acc[b][1]);
-                nxt[b][r >> 1][(r & 1) * 2 + 1] = gelu_pair(acc[b][2], acc[b][3]);
+                nxt[b][(RB0 + r) >> 1][((RB0 + r) & 1) * 2] =
+                    gelu_pair(acc[b][0], acc[b][1]);
+                nxt[b][(RB0 + r) >> 1][((RB0 + r) & 1) * 2 + 1] =
+                    gelu_pair(acc[b][2], acc[b][3]);
             } else {
-                res[b][r] = acc[b];
+                res[b][RB0 + r] = acc[b];
             }
         }
 #pragma unroll
@@ -757,15 +769,29 @@ __global__ __launch_bounds__(ITHREADS) void k_mlp_infer(
     __syncthreads();
     infer_layer<IKIN / 32, IH / 16, true>(lt, bias, act, nxt, res, lane);
     src += n0;
+    // the 256 x 256 layers in two halves of 8 row blocks each
+    constexpr int nh = n1 / IPIECES;
+    static_assert(IPIECES == 2, "two halves per hidden layer");
     __syncthreads();
-    stage_tiles<n1>(src, (uint4 *)smem);
+    stage_tiles<nh>(src, (uint4 *)smem);
     __syncthreads();
-    infer_layer<IH / 32, IH / 16, true>(lt, bias + IH, nxt, act, res, lane);
+    infer_layer<IH / 32, IH / 32, true, 0>(lt, bias + IH, nxt, act, res, lane);
+    __syncthreads();
+    stage_tiles<nh>(src + nh, (uint4 *)smem);
+    __syncthreads();
+    infer_layer<IH / 32, IH / 32, true, IH / 32>(lt, bias + IH, nxt, act, res,
+                                                 lane);
     src += n1;
     __syncthreads();
-    stage_tiles<n1>(src, (uint4 *)smem);
+    stage_tiles<nh>(src, (uint4 *)smem);
     __syncthreads();
-    infer_layer<IH / 32, IH / 16, true>(lt, bias + 2 * IH, act, nxt, res, lane);
+    infer_layer<IH / 32, IH / 32, true, 0>(lt, bias + 2 * IH, act, nxt, res,
+                                           lane);
+    __syncthreads();
+    stage_tiles<nh>(src + nh, (uint4 *)smem);
+    __syncthreads();
+    infer_layer<IH / 32, IH / 32, true, IH / 32>(lt, bias + 2 * IH, act, nxt,
+                                                 res, lane);
     src += n1;
     __syncthreads();
     stage_tiles<n3>(src, (uint4 *)smem);
