@@ -1617,8 +1617,9 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
     fexp = 'crs::exp64' if opts.get('exp') == 'crs' else 'exp'
     # the mixed kernel's arithmetic (mixed_arith): 'accurate' (default)
     # keeps Kc (the G_k), the net rate and the sums in full FP64 and takes
-    # the Arrhenius and falloff transcendentals at about 32 bits (degree-5
-    # exponential, crs::exp2_m64; falloff blocks in FP64 arithmetic);
+    # the Arrhenius and falloff exponentials at about 27 bits (degree-4
+    # exponential, crs::exp2_m64, 1.0e-8 relative; falloff and third-body
+    # blocks in FP64 arithmetic);
     # 'fp32' is the FP32 variant (hardware v_exp_f32 cores, FP32 falloff
     # and third-body blocks), faster and about 20x less accurate
     m32 = mixed and opts.get('mixed_arith', 'accurate') == 'fp32'
