@@ -122,38 +122,40 @@ __device__ __forceinline__ double exp64(double x)
     return __builtin_ldexp(__builtin_fma(r, q, 1.0), (int)n);
 }
 
-// 2^t in FP64, same polynomial: t = n + s with n = rint(t) and s = t - n
-// exact, r = s ln2 (one rounding, |r| <= ln2/2), then as exp64 (6.2 ulp
-// max).  For exponents formed in base 2 (constants pre-scaled by log2 e),
-// which saves exp64's conversion multiply and its two-step reduction.
+// 2^t in FP64 to about 6 ulp (5.9 * 2^-53 max by exact emulation): t = n +
+// s with n = rint(t) and s = t - n exact, |s| <= 1/2; 2^s = 1 + s Q2(s)
+// where Q2(s) = (2^s - 1) / s is the degree-9 Chebyshev interpolant on
+// [-1/2, 1/2] (exp64_coeffs.py coefficients_b2), so the reduced argument
+// needs no conversion to base e.  For exponents formed in base 2 (constants
+// pre-scaled by log2 e): no conversion multiply, no two-step reduction.
 __device__ __forceinline__ double exp2_64(double t)
 {
     const double n = __builtin_rint(t);
-    const double r = (t - n) * 0.6931471805599453;
-    double q = 2.7620088445409746e-07;
-    q = __builtin_fma(q, r, 2.763265575933203e-06);
-    q = __builtin_fma(q, r, 2.4801521295954376e-05);
-    q = __builtin_fma(q, r, 0.0001984119061583585);
-    q = __builtin_fma(q, r, 0.0013888888917213717);
-    q = __builtin_fma(q, r, 0.008333333367332009);
-    q = __builtin_fma(q, r, 0.04166666666662413);
-    q = __builtin_fma(q, r, 0.16666666666615607);
-    q = __builtin_fma(q, r, 0.5000000000000001);
-    q = __builtin_fma(q, r, 1.0000000000000013);
-    return __builtin_ldexp(__builtin_fma(r, q, 1.0), (int)n);
+    const double s = t - n;
+    double q = 7.070981083365768e-09;
+    q = __builtin_fma(q, s, 1.0205910984889862e-07);
+    q = __builtin_fma(q, s, 1.3215451619146607e-06);
+    q = __builtin_fma(q, s, 1.52526729004753e-05);
+    q = __builtin_fma(q, s, 0.0001540353042479538);
+    q = __builtin_fma(q, s, 0.001333355820082724);
+    q = __builtin_fma(q, s, 0.009618129107618658);
+    q = __builtin_fma(q, s, 0.05550410866465154);
+    q = __builtin_fma(q, s, 0.24022650695910075);
+    q = __builtin_fma(q, s, 0.6931471805599462);
+    return __builtin_ldexp(__builtin_fma(s, q, 1.0), (int)n);
 }
 
-// The constants of exp64 / exp2_64 as run-time values held in registers.
-// A fused kernel takes them once at its start (ExpK k = exp_consts()) and
-// passes them to exp64k / exp2_64k: each reaction block is its own basic
-// block, and with literal coefficients every block would materialise all of
-// them again (two scalar moves per constant, plus a vector move for the one
-// that must sit in a VGPR: a VALU instruction takes one scalar operand).
-// The values pass through empty asm so the compiler cannot fold them back
-// into literals; e8 is kept in a VGPR pair for the first Horner step.
+// The constants of exp2_64 as run-time values held in registers.  A fused
+// kernel takes them once at its start (ExpK k = exp_consts()) and passes
+// them to exp2_64k / exp64k: each reaction block is its own basic block,
+// and with literal coefficients every block would materialise all of them
+// again (two scalar moves per constant, plus a vector move for the one that
+// must sit in a VGPR: a VALU instruction takes one scalar operand).  The
+// values pass through empty asm so the compiler cannot fold them back into
+// literals; b8 is kept in a VGPR pair for the first Horner step.
 struct ExpK {
-    double e9, e8, e7, e6, e5, e4, e3, e2, e1, e0;
-    double ln2, ln2lo, log2e;
+    double b9, b8, b7, b6, b5, b4, b3, b2, b1, b0;
+    double log2e;
 };
 
 __device__ __forceinline__ double sgpr_const(double x)
@@ -171,61 +173,57 @@ __device__ __forceinline__ double vgpr_const(double x)
 __device__ __forceinline__ ExpK exp_consts()
 {
     ExpK k;
-    k.e9 = sgpr_const(2.7620088445409746e-07);
-    k.e8 = vgpr_const(2.763265575933203e-06);
-    k.e7 = sgpr_const(2.4801521295954376e-05);
-    k.e6 = sgpr_const(0.0001984119061583585);
-    k.e5 = sgpr_const(0.0013888888917213717);
-    k.e4 = sgpr_const(0.008333333367332009);
-    k.e3 = sgpr_const(0.04166666666662413);
-    k.e2 = sgpr_const(0.16666666666615607);
-    k.e1 = sgpr_const(0.5000000000000001);
-    k.e0 = sgpr_const(1.0000000000000013);
-    k.ln2 = sgpr_const(0.6931471805599453);
-    k.ln2lo = sgpr_const(2.3190468138462996e-17);
+    k.b9 = sgpr_const(7.070981083365768e-09);
+    k.b8 = vgpr_const(1.0205910984889862e-07);
+    k.b7 = sgpr_const(1.3215451619146607e-06);
+    k.b6 = sgpr_const(1.52526729004753e-05);
+    k.b5 = sgpr_const(0.0001540353042479538);
+    k.b4 = sgpr_const(0.001333355820082724);
+    k.b3 = sgpr_const(0.009618129107618658);
+    k.b2 = sgpr_const(0.05550410866465154);
+    k.b1 = sgpr_const(0.24022650695910075);
+    k.b0 = sgpr_const(0.6931471805599462);
     k.log2e = sgpr_const(1.4426950408889634);
     return k;
 }
 
-// exp64 / exp2_64 on the register-held constants: the same operations in
-// the same order, so the same results bit for bit.
-__device__ __forceinline__ double exp_tail(const ExpK &k, double r, double n)
-{
-    double q = __builtin_fma(k.e9, r, k.e8);
-    q = __builtin_fma(q, r, k.e7);
-    q = __builtin_fma(q, r, k.e6);
-    q = __builtin_fma(q, r, k.e5);
-    q = __builtin_fma(q, r, k.e4);
-    q = __builtin_fma(q, r, k.e3);
-    q = __builtin_fma(q, r, k.e2);
-    q = __builtin_fma(q, r, k.e1);
-    q = __builtin_fma(q, r, k.e0);
-    return __builtin_ldexp(__builtin_fma(r, q, 1.0), (int)n);
-}
-
-__device__ __forceinline__ double exp64k(const ExpK &k, double x)
-{
-    const double n = __builtin_rint(x * k.log2e);
-    double r = __builtin_fma(-n, k.ln2, x);
-    r = __builtin_fma(-n, k.ln2lo, r);
-    return exp_tail(k, r, n);
-}
-
+// exp2_64 on the register-held constants: the same operations in the same
+// order, so the same results bit for bit.
 __device__ __forceinline__ double exp2_64k(const ExpK &k, double t)
 {
     const double n = __builtin_rint(t);
-    const double r = (t - n) * k.ln2;
-    return exp_tail(k, r, n);
+    const double s = t - n;
+    double q = __builtin_fma(k.b9, s, k.b8);
+    q = __builtin_fma(q, s, k.b7);
+    q = __builtin_fma(q, s, k.b6);
+    q = __builtin_fma(q, s, k.b5);
+    q = __builtin_fma(q, s, k.b4);
+    q = __builtin_fma(q, s, k.b3);
+    q = __builtin_fma(q, s, k.b2);
+    q = __builtin_fma(q, s, k.b1);
+    q = __builtin_fma(q, s, k.b0);
+    return __builtin_ldexp(__builtin_fma(s, q, 1.0), (int)n);
 }
 
-// Exponentials of the mixed-precision kernel: the same reduction as exp64 /
-// exp2_64 (in FP64: n = rint(t), r = (t - n) ln2), then e^r = 1 + r (1 +
-// r q(r)) with q of degree 4 instead of 9: relative error 1.0e-8 (exact
-// emulation in exp64_coeffs.py), five FMAs fewer.  FP64 arithmetic at
-// about 27 bits of accuracy, better than the FP32 hardware exponential
-// (about 1e-7, which a species sum with cancellation amplifies up to
-// 20-fold) and about as cheap once the conversions and the quarter issue
-// rate of v_exp_f32 are counted.
+// e^x in the fused kernels as 2^(x log2 e): the rounding of the product
+// adds |x| 2^-53 to the relative error (102 * 2^-53 at |x| <= 100, exact
+// emulation in exp64_coeffs.py).  The emitter pre-scales the arguments it
+// forms itself (NASA Gibbs sums, Troe and Arrhenius exponents) and calls
+// exp2_64k directly; this form is left for the rarer rate terms (SRI
+// blending, d T^e).
+__device__ __forceinline__ double exp64k(const ExpK &k, double x)
+{
+    return exp2_64k(k, x * k.log2e);
+}
+
+// Exponentials of the mixed-precision kernel: the same reduction as
+// exp2_64 (n = rint(t), s = t - n exact), then 2^s = 1 + s (ln2 + s q2(s))
+// with q2(s) = (2^s - 1 - s ln2) / s^2 of degree 4 (exp64_coeffs.py
+// coefficients_b2m): relative error 1.0e-8 by exact emulation, five FMAs
+// fewer than exp2_64.  FP64 arithmetic at about 27 bits of accuracy, better
+// than the FP32 hardware exponential (about 1e-7, which a species sum with
+// cancellation amplifies up to 20-fold) and about as cheap once the
+// conversions and the quarter issue rate of v_exp_f32 are counted.
 struct ExpMK {
     double c4, c3, c2, c1, c0, ln2, log2e;
 };
@@ -233,11 +231,11 @@ struct ExpMK {
 __device__ __forceinline__ ExpMK exp_m_consts()
 {
     ExpMK k;
-    k.c4 = sgpr_const(0.0013926183587713494);
-    k.c3 = vgpr_const(0.008363179052722578);
-    k.c2 = sgpr_const(0.041666554617204574);
-    k.c1 = sgpr_const(0.16666576989698775);
-    k.c0 = sgpr_const(0.5);
+    k.c4 = sgpr_const(0.000154448922353153);
+    k.c3 = vgpr_const(0.001338131210261626);
+    k.c2 = sgpr_const(0.00961810324267985);
+    k.c1 = sgpr_const(0.05550381001841135);
+    k.c0 = sgpr_const(0.24022650695910072);
     k.ln2 = sgpr_const(0.6931471805599453);
     k.log2e = sgpr_const(1.4426950408889634);
     return k;
@@ -246,12 +244,12 @@ __device__ __forceinline__ ExpMK exp_m_consts()
 __device__ __forceinline__ double exp2_m64k(const ExpMK &k, double t)
 {
     const double n = __builtin_rint(t);
-    const double r = (t - n) * k.ln2;
-    double q = __builtin_fma(k.c4, r, k.c3);
-    q = __builtin_fma(q, r, k.c2);
-    q = __builtin_fma(q, r, k.c1);
-    q = __builtin_fma(q, r, k.c0);
-    const double p = __builtin_fma(r, __builtin_fma(r, q, 1.0), 1.0);
+    const double s = t - n;
+    double q = __builtin_fma(k.c4, s, k.c3);
+    q = __builtin_fma(q, s, k.c2);
+    q = __builtin_fma(q, s, k.c1);
+    q = __builtin_fma(q, s, k.c0);
+    const double p = __builtin_fma(s, __builtin_fma(s, q, k.ln2), 1.0);
     return __builtin_ldexp(p, (int)n);
 }
 

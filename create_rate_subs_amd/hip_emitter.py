@@ -512,12 +512,13 @@ def _cheb_lines(x, K, P='P'):
     return out
 
 
-def _fast_exponent(a, K):
+def _fast_exponent(a, K, log2=False):
     """ln of an Arrhenius expression (lnA + b lnT - E/T with 1/T
-    precomputed; the log of the literal forms without an exponential)."""
+    precomputed; the log of the literal forms without an exponential);
+    log2 of it with ``log2``."""
     if not a.has_exp or a.pre is not None:
-        return 'log({})'.format(_arr_expr(a, K))
-    lines, xname = _fast_arr(a, K, 'tmp')
+        return ('log2({})' if log2 else 'log({})').format(_arr_expr(a, K))
+    lines, xname = _fast_arr(a, K, 'tmp', log2=log2)
     return lines[0][len('const double x_tmp = '):-1]
 
 
@@ -527,7 +528,12 @@ def _plog_fast_lines(x, K, expfn):
     exponential is taken, same thresholds as the reference."""
     pl = x.plog
     n = len(pl['p'])
-    xs = [_fast_exponent(a, K) for a in pl['arr']]
+    # with the branch-free exponential the rows' exponents are base 2 (the
+    # interpolation is linear, so xk is too) and kf = crs::exp2_64(xk)
+    base2 = expfn == 'crs::exp64'
+    if base2:
+        expfn = 'crs::exp2_64'
+    xs = [_fast_exponent(a, K, log2=base2) for a in pl['arr']]
     out = ['double xk;',
            'if (P <= {}) {{'.format(_lit(pl['p'][0])),
            '    xk = {};'.format(xs[0])]
@@ -570,7 +576,11 @@ def _cheb_fast_lines(x, K, expfn):
         out.append('tt{o} = 2.0 * Tred * tt{n} - tt{o};'.format(o=old, n=new))
         out.append('lk += d{} * tt{};'.format(i, old))
         one = not one
-    out.append('const double kf = {}(lk * 2.302585092994046);'.format(expfn))
+    if expfn == 'crs::exp64':
+        out.append('const double kf = crs::exp2_64(lk * {});'.format(_LOG2_10))
+    else:
+        out.append('const double kf = {}(lk * 2.302585092994046);'.format(
+            expfn))
     return out
 
 
@@ -951,20 +961,24 @@ _LOG2_10 = '3.321928094887362'
 
 
 def _exp_t(pos, c, K):
-    """exp(-T c) (exp(T c) when not pos): crs::exp64 while |T c| stays
-    below 1e8 for T up to 10^4 K (exp64's reduction is exact there and
-    it over/underflows like exp beyond 709), the library exp otherwise
-    (e.g. T*** = 1e-30 K)."""
-    fn = 'crs::exp64' if abs(c) * 1.0e4 <= 1.0e8 else 'exp'
-    return '{}({}T * {})'.format(fn, '-' if pos else '', K(c))
+    """exp(-T c) (exp(T c) when not pos): crs::exp2_64 on the constant
+    pre-scaled by log2 e while |T c| stays below 1e8 for T up to 10^4 K
+    (the reduction is exact there and 2^t over/underflows like exp beyond
+    709), the library exp otherwise (e.g. T*** = 1e-30 K)."""
+    if abs(c) * 1.0e4 <= 1.0e8:
+        return 'crs::exp2_64({}T * {})'.format('-' if pos else '',
+                                               K(c * _LOG2E))
+    return 'exp({}T * {})'.format('-' if pos else '', K(c))
 
 
 def _exp_it(pos, c, K):
-    """exp(-c / T) as exp(-c * iT); exp64 while |c| / T stays below 1e8
-    for T down to 1 K (and below 7e4 so that the term cannot underflow to
-    zero at T >= 100 K)."""
-    fn = 'crs::exp64' if abs(c) <= 7.0e4 else 'exp'
-    return '{}({}{} * iT)'.format(fn, '-' if pos else '', K(c))
+    """exp(-c / T) as 2^(-(c log2 e) iT); crs::exp2_64 while |c| / T stays
+    below 1e8 for T down to 1 K (and below 7e4 so that the term cannot
+    underflow to zero at T >= 100 K), the library exp otherwise."""
+    if abs(c) <= 7.0e4:
+        return 'crs::exp2_64({}{} * iT)'.format('-' if pos else '',
+                                                K(c * _LOG2E))
+    return 'exp({}{} * iT)'.format('-' if pos else '', K(c))
 
 
 def _fast_pm_lines(x, K, kf_x, arr2=None):
@@ -1095,14 +1109,16 @@ def _gibbs_species(ir):
     return sorted(need)
 
 
-def _nasa_table_values(ir):
+def _nasa_table_values(ir, scale=1.0):
     """Per species, per range (lower, upper): the 7 coefficients of
-    s/R - h/RT on [1, ln T, T, T^2, T^3, T^4, 1/T], padded to 8."""
+    s/R - h/RT on [1, ln T, T, T^2, T^3, T^4, 1/T], padded to 8, each
+    multiplied by ``scale`` (log2 e for a base-2 exponential)."""
     vals = []
     for _tm, lo, hi in ir.nasa:
         for a in (lo, hi):
-            vals += [a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0, a[3] / 12.0,
-                     a[4] / 20.0, -a[5], 0.0]
+            vals += [v * scale for v in
+                     (a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0, a[3] / 12.0,
+                      a[4] / 20.0, -a[5])] + [0.0]
     return vals
 
 
@@ -1112,8 +1128,15 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False,
 
     With them Kc = prod_k G_k^nu_k (PA / (RU T))^dnu for every reversible
     reaction, so a reaction needs a few products instead of its own
-    exponential of a lumped NASA polynomial.
+    exponential of a lumped NASA polynomial.  With the branch-free
+    exponential (expfn crs::exp64) the NASA coefficients are pre-scaled by
+    log2 e and G_k = crs::exp2_64(smh_k): no conversion multiply per species
+    (the LDS table, nasa_lds, holds the scaled values likewise).
     """
+    base2 = expfn == 'crs::exp64'
+    sc = _LOG2E if base2 else 1.0
+    if base2:
+        expfn = 'crs::exp2_64'
     need_g, need_gi, dnus = set(), set(), set()
     for x in ir.rxns:
         if not _species_kc_ok(x):
@@ -1167,8 +1190,9 @@ def _species_gibbs_lines(ir, K, tmids, store_inverse=False,
         else:
             terms = []
             for half, a in (('l', sp[1]), ('h', sp[2])):
-                coef = [a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0,
-                        a[3] / 12.0, a[4] / 20.0, -a[5]]
+                coef = [v * sc for v in
+                        (a[6] - a[0], a[0], a[1] / 2.0, a[2] / 6.0,
+                         a[3] / 12.0, a[4] / 20.0, -a[5])]
                 for bi, cv in enumerate(coef):
                     if cv != 0.0:
                         terms.append('{} * p{}{}_{}'.format(K(cv), half, gi,
@@ -2168,7 +2192,14 @@ def emit_hip_source(ir, opts=None):
     # ---- NASA coefficient table in LDS (option nasa='lds') -------------
     nasa_fill = ''
     if 'lds' in (opts.get('nasa'), dopts.get('nasa')):
-        vals = _nasa_table_values(ir)
+        # the table holds s/R - h/RT in the units of the exponential that
+        # reads it: log2 e times the coefficients for crs::exp2_64
+        base2 = {o.get('exp') == 'crs' for o in (opts, dopts)
+                 if o.get('nasa') == 'lds'}
+        if len(base2) > 1:
+            raise ValueError("the NASA table is shared by the wdot and dydt "
+                             "kernels: give both the same 'exp' option")
+        vals = _nasa_table_values(ir, _LOG2E if base2.pop() else 1.0)
         body.append('__device__ const double crs_nasa_tab[{}] = {{\n'.format(
             len(vals)) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
                                                else ' ')
