@@ -155,7 +155,7 @@ __device__ __forceinline__ double exp2_64(double t)
 // literals; b8 is kept in a VGPR pair for the first Horner step.
 struct ExpK {
     double b9, b8, b7, b6, b5, b4, b3, b2, b1, b0;
-    double log2e;
+    double log2e, sqrt2;
 };
 
 __device__ __forceinline__ double sgpr_const(double x)
@@ -184,6 +184,7 @@ __device__ __forceinline__ ExpK exp_consts()
     k.b1 = sgpr_const(0.24022650695910075);
     k.b0 = sgpr_const(0.6931471805599462);
     k.log2e = sgpr_const(1.4426950408889634);
+    k.sqrt2 = sgpr_const(1.4142135623730951);
     return k;
 }
 
@@ -216,6 +217,43 @@ __device__ __forceinline__ double exp64k(const ExpK &k, double x)
     return exp2_64k(k, x * k.log2e);
 }
 
+// log2 x for positive normal x (callers clamp with fmax first), to about
+// 1e-15 absolute (exact emulation in log64_coeffs.py with the hardware
+// seed taken 2^-20 off).  x = 2^e m, m in [1/2, 1); the hardware FP32
+// log2 of m gives y0 within about 2^-22; s = -y0 - 1/2 (exact, |s| <= 1/2),
+// 2^-y0 = sqrt2 * 2^s from the exp2_64 polynomial, and the residual
+// t = m 2^-y0 - 1 (|t| < 1e-6) corrects y0 by log2(1 + t) =
+// (t - t^2/2) log2 e to within t^3.  About 23 VALU instructions against
+// log64's 34; the error is absolute (the falloff blocks' log10 Pr and
+// log10 Fcent enter sums, where absolute error is what counts), so it is
+// not a substitute for log64 near x = 1 where a relative bound matters.
+// +inf gives NaN.
+__device__ __forceinline__ double log2_64k(const ExpK &k, double x)
+{
+    const double m = __builtin_amdgcn_frexp_mant(x);
+    const int e = __builtin_amdgcn_frexp_exp(x);
+    const double y0 = (double)__builtin_amdgcn_logf((float)m);
+    const double s = -y0 - 0.5;
+    double q = __builtin_fma(k.b9, s, k.b8);
+    q = __builtin_fma(q, s, k.b7);
+    q = __builtin_fma(q, s, k.b6);
+    q = __builtin_fma(q, s, k.b5);
+    q = __builtin_fma(q, s, k.b4);
+    q = __builtin_fma(q, s, k.b3);
+    q = __builtin_fma(q, s, k.b2);
+    q = __builtin_fma(q, s, k.b1);
+    q = __builtin_fma(q, s, k.b0);
+    const double p = __builtin_fma(s, q, 1.0);
+    const double t = __builtin_fma(m * k.sqrt2, p, -1.0);
+    const double u = __builtin_fma(-0.5 * t, t, t);
+    return __builtin_fma(u, k.log2e, y0) + (double)e;
+}
+
+__device__ __forceinline__ double log2_64(double x)
+{
+    return log2_64k(exp_consts(), x);
+}
+
 // Exponentials of the mixed-precision kernel: the same reduction as
 // exp2_64 (n = rint(t), s = t - n exact), then 2^s = 1 + s (ln2 + s q2(s))
 // with q2(s) = (2^s - 1 - s ln2) / s^2 of degree 4 (exp64_coeffs.py
@@ -225,7 +263,7 @@ __device__ __forceinline__ double exp64k(const ExpK &k, double x)
 // cancellation amplifies up to 20-fold) and about as cheap once the
 // conversions and the quarter issue rate of v_exp_f32 are counted.
 struct ExpMK {
-    double c4, c3, c2, c1, c0, ln2, log2e;
+    double c4, c3, c2, c1, c0, ln2, log2e, sqrt2;
 };
 
 __device__ __forceinline__ ExpMK exp_m_consts()
@@ -238,6 +276,7 @@ __device__ __forceinline__ ExpMK exp_m_consts()
     k.c0 = sgpr_const(0.24022650695910072);
     k.ln2 = sgpr_const(0.6931471805599453);
     k.log2e = sgpr_const(1.4426950408889634);
+    k.sqrt2 = sgpr_const(1.4142135623730951);
     return k;
 }
 
@@ -268,6 +307,30 @@ __device__ __forceinline__ double exp2_m64(double t)
 __device__ __forceinline__ double exp_m64(double x)
 {
     return exp_m64k(exp_m_consts(), x);
+}
+
+// log2_64k at the mixed kernel's accuracy: the same residual correction on
+// the degree-4 exp2 form, about 1.5e-8 absolute (log64_coeffs.py), five
+// FMAs fewer.
+__device__ __forceinline__ double log2_m64k(const ExpMK &k, double x)
+{
+    const double m = __builtin_amdgcn_frexp_mant(x);
+    const int e = __builtin_amdgcn_frexp_exp(x);
+    const double y0 = (double)__builtin_amdgcn_logf((float)m);
+    const double s = -y0 - 0.5;
+    double q = __builtin_fma(k.c4, s, k.c3);
+    q = __builtin_fma(q, s, k.c2);
+    q = __builtin_fma(q, s, k.c1);
+    q = __builtin_fma(q, s, k.c0);
+    const double p = __builtin_fma(s, __builtin_fma(s, q, k.ln2), 1.0);
+    const double t = __builtin_fma(m * k.sqrt2, p, -1.0);
+    const double u = __builtin_fma(-0.5 * t, t, t);
+    return __builtin_fma(u, k.log2e, y0) + (double)e;
+}
+
+__device__ __forceinline__ double log2_m64(double x)
+{
+    return log2_m64k(exp_m_consts(), x);
 }
 
 // exp2_64 / exp64 in the q form, e^r = 1 + r (1 + r q(r)) with

@@ -108,8 +108,94 @@ def max_error(samples=20000, seed=3):
     return worst / 2.0 ** -53
 
 
+def _exp_coeffs():
+    """The exp2 polynomials of exp64_coeffs.py beside this file."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        'exp64_coeffs', os.path.join(os.path.dirname(os.path.abspath(
+            __file__)), 'exp64_coeffs.py'))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+_POLY = {}
+
+
+def _exp2_poly(mixed):
+    """exp2_64's (mixed: exp2_m64k's) coefficients, derived once."""
+    if mixed not in _POLY:
+        ec = _exp_coeffs()
+        _POLY[mixed] = (ec.coefficients_b2m() if mixed else
+                        ec.coefficients_b2())
+    return _POLY[mixed]
+
+
+SQRT2 = 1.4142135623730951
+LOG2E = 1.4426950408889634
+
+
+def emulate_log2(x, seed_off=0.0, mixed=False, cs=None, second_order=True):
+    """crs::log2_64k (mixed: crs::log2_m64k) with every double operation
+    rounded once.  The hardware FP32 log2 of the mantissa is modelled as
+    the FP32 rounding of log2(fl32(m)) moved by ``seed_off`` (absolute) and
+    rounded to FP32 again: the sequence must not depend on that seed's last
+    bits."""
+    import numpy as np
+
+    def rd(v):
+        return mp.mpf(float(v))
+
+    def fma(a, b, c):
+        return rd(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+    if cs is None:
+        cs = _exp2_poly(mixed)
+    m, e = mp.frexp(mp.mpf(x))
+    mf = float(np.float32(float(m)))
+    y0 = float(np.float32(float(mp.log(mf, 2))))
+    y0 = mp.mpf(float(np.float32(y0 + seed_off)))
+    s = -y0 - mp.mpf(0.5)                      # exact in double
+    q = mp.mpf(cs[0])
+    for c in cs[1:]:
+        q = fma(q, s, c)
+    if mixed:
+        p = fma(s, fma(s, q, float(mp.log(2))), 1.0)
+    else:
+        p = fma(s, q, 1.0)
+    t = fma(rd(m * SQRT2), p, -1.0)
+    u = fma(rd(-0.5 * t), t, t) if second_order else t
+    return rd(fma(u, LOG2E, y0) + e)
+
+
+def max_error_log2(samples=4000, seed=5, mixed=False, cs=None,
+                   second_order=True):
+    """Max error of the emulated log2_64k (log2_m64k) against log2 x over
+    1e-300 .. 1e300 and next to powers of two, with the FP32 seed exact and
+    2^-20 off in either direction: absolute where |log2 x| < 1, relative
+    beyond (the final sum e + log2 m rounds to the ulp of its size)."""
+    rng = random.Random(seed)
+    worst = 0.0
+    for i in range(samples):
+        k = i % 3
+        if k == 0:
+            x = 10.0 ** rng.uniform(-300.0, 300.0)
+        elif k == 1:
+            x = rng.uniform(0.25, 4.0)
+        else:
+            x = 2.0 ** rng.randint(-60, 60) * (1.0 + rng.uniform(-1e-6, 1e-6))
+        ex = mp.log(mp.mpf(x), 2)
+        for so in (0.0, 2.0 ** -20, -(2.0 ** -20)):
+            got = emulate_log2(x, so, mixed, cs, second_order)
+            worst = max(worst, float(abs(got - ex) / max(abs(ex), 1)))
+    return worst
+
+
 if __name__ == '__main__':
     print('P coefficients:', [repr(c) for c in coefficients()])
     print('ln2 split:', [repr(v) for v in ln2_split()])
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 20000
     print('log64 max error (units of 2^-53):', max_error(n))
+    print('log2_64k max absolute error:', max_error_log2(n // 5))
+    print('log2_m64k max absolute error:', max_error_log2(n // 5, mixed=True))

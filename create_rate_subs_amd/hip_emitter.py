@@ -958,6 +958,7 @@ def _fast_lnkc(x, K):
 
 _LOG10E = '0.4342944819032518'
 _LOG2_10 = '3.321928094887362'
+_LOG10_2 = '0.3010299956639812'
 
 
 def _exp_t(pos, c, K):
@@ -1028,16 +1029,20 @@ def _fast_pm_lines(x, K, kf_x, arr2=None):
         if t['t2'] is not None:
             fc += ' + ' + _exp_it(t['t2_pos'], t['t2'], K)
         # Fcent <= 3 is finite; an infinite Pr makes pm NaN in the reference
-        # too (inf / inf in A^2 / B^2), so neither log needs the inf guard
-        out.append('const double logFcent = crs::log64_finite(fmax({}, '
-                   '1.0e-300)) * {};'.format(fc, _LOG10E))
-        out.append('const double lPr = crs::log64_finite(fmax(Pr, 1.0e-300)) '
-                   '* {};'.format(_LOG10E))
+        # too (inf / inf in A^2 / B^2), so neither log needs the inf guard.
+        # Both logs in base 2 (crs::log2_64: the hardware seed refined on the
+        # exp2 polynomial, absolute error ~1e-15, which is what A and B
+        # need), and 10^(logFcent / (1 + A^2 / B^2)) =
+        # 2^(log2 Fcent B^2 / (B^2 + A^2))
+        out.append('const double l2Fc = crs::log2_64(fmax({}, 1.0e-300));'
+                   .format(fc))
+        out.append('const double logFcent = l2Fc * {};'.format(_LOG10_2))
+        out.append('const double lPr = crs::log2_64(fmax(Pr, 1.0e-300)) * '
+                   '{};'.format(_LOG10_2))
         out.append('const double A = lPr - 0.67 * logFcent - 0.4;')
         out.append('const double B = 0.806 - 1.1762 * logFcent - 0.14 * lPr;')
-        # 10^(logFcent / (1 + A^2 / B^2)) = 2^(log2(10) logFcent B^2 / (B^2 + A^2))
-        out.append('const double pm = crs::exp2_64(logFcent * (B * B) * '
-                   'crs::rcp64(B * B + A * A) * {}){};'.format(_LOG2_10, tail))
+        out.append('const double pm = crs::exp2_64(l2Fc * (B * B) * '
+                   'crs::rcp64(B * B + A * A)){};'.format(tail))
     elif x.falloff == 'sri':
         sr = x.sri
         out.append('const double lPr = crs::log64(fmax(Pr, 1.0e-300)) * '
@@ -1840,6 +1845,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
         lines = lines[:rx_start] + [
             l.replace('crs::exp64(', 'crs::exp_m64(')
              .replace('crs::exp2_64(', 'crs::exp2_m64(')
+             .replace('crs::log2_64(', 'crs::log2_m64(')
             for l in lines[rx_start:]]
     if dydt:
         # energy equation, rate_subs.py:1910-1925
@@ -1909,8 +1915,10 @@ def _split_blocks(lines, per):
     lines = ['const crs::ExpK ek = crs::exp_consts();'] + [
         l.replace('crs::exp2_64(', 'crs::exp2_64k(ek, ')
          .replace('crs::exp64(', 'crs::exp64k(ek, ')
+         .replace('crs::log2_64(', 'crs::log2_64k(ek, ')
          .replace('crs::exp2_m64(', 'crs::exp2_m64k(ekm, ')
-         .replace('crs::exp_m64(', 'crs::exp_m64k(ekm, ') for l in lines]
+         .replace('crs::exp_m64(', 'crs::exp_m64k(ekm, ')
+         .replace('crs::log2_m64(', 'crs::log2_m64k(ekm, ') for l in lines]
     if any('ekm, ' in l for l in lines):
         lines.insert(1, 'const crs::ExpMK ekm = crs::exp_m_consts();')
     # segments: ('plain', [lines]) outside groups, ('group', [lines])

@@ -90,3 +90,48 @@ def test_pow64_composition_bound():
         ex = mp.power(mp.mpf(x), mp.mpf(y))
         worst = max(worst, float(abs(got - ex) / ex))
     assert worst < 1e-13
+
+
+def _fn_body(signature):
+    text = open(os.path.join(CSRC, 'crs_common.h')).read()
+    body = text[text.index(signature):]
+    return body[:body.index('\n}\n')]
+
+
+def test_log2_64k_uses_the_exp2_polynomial_and_bound():
+    """crs::log2_64k (the falloff blocks' logarithm) refines the hardware
+    FP32 log2 seed with the register-held exp2 polynomial; emulated with
+    the seed exact or 2^-20 off, it stays within 12 units of 2^-53 (absolute
+    below |log2 x| = 1, relative above)."""
+    mod = _coeffs_module()
+    body = _fn_body('double log2_64k(const ExpK &k, double x)')
+    assert ['k.b{}'.format(i) in body for i in range(10)] == [True] * 10
+    assert 'k.sqrt2' in body and 'k.log2e' in body
+    assert '__builtin_amdgcn_logf' in body
+    text = open(os.path.join(CSRC, 'crs_common.h')).read()
+    consts = text[text.index('ExpK exp_consts()'):]
+    assert 'k.sqrt2 = sgpr_const(1.4142135623730951);' in consts
+    assert mod.SQRT2 == float(mp.sqrt(2))
+    assert mod.max_error_log2(samples=600, seed=13) <= 12 * 2.0 ** -53
+
+
+def test_log2_m64k_bound():
+    """The mixed kernel's log2 (degree-4 exp2 form) stays within 2e-8."""
+    mod = _coeffs_module()
+    body = _fn_body('double log2_m64k(const ExpMK &k, double x)')
+    assert 'k.c4' in body and 'k.sqrt2' in body
+    assert mod.max_error_log2(samples=600, seed=14, mixed=True) <= 2e-8
+
+
+def test_log2_planted_mistakes_break_the_bound():
+    """Without the t^2/2 term, or with sqrt 2 off in its 9th digit, the
+    correction no longer absorbs a seed 2^-20 off."""
+    mod = _coeffs_module()
+    assert mod.max_error_log2(samples=60, seed=15,
+                              second_order=False) > 12 * 2.0 ** -53
+    good = mod.SQRT2
+    try:
+        mod.SQRT2 = 1.41421356
+        assert mod.max_error_log2(samples=60, seed=16) > 12 * 2.0 ** -53
+    finally:
+        mod.SQRT2 = good
