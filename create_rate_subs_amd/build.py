@@ -2,18 +2,25 @@
 
 ``compile_library(src, out_dir)`` runs hipcc on one ``crs_<name>.hip`` and
 writes ``libcrs_<name>.so`` next to it.  ``build_mechanism(name)`` emits and
-compiles one of the synthetic mechanisms into ``_build/<name>/``.
+compiles one of the synthetic mechanisms into ``_build/<name>/``;
+``build_with_groups`` picks, per fused kernel, the reaction-group size the
+register allocator spills least at (see its docstring).
 """
 
+import json
 import os
+import re
 import subprocess
+import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 from . import hip_emitter, workloads
 from .create_rate_subs import load_mechanism
 
 __all__ = ['HIPCC_FLAGS', 'compile_library', 'build_mechanism', 'lib_path',
-           'INCLUDE_DIRS', 'build_mlp']
+           'INCLUDE_DIRS', 'build_mlp', 'build_with_groups',
+           'parse_resource_usage', 'GROUP_CANDIDATES']
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -30,12 +37,20 @@ def hipcc():
     return os.path.join(rocm, 'bin', 'hipcc')
 
 
-def compile_library(src, out_dir=None, extra_flags=(), verbose=False):
-    """hipcc ``src`` -> ``libcrs_<name>.so``; return the library path."""
+_REMARK = '-Rpass-analysis=kernel-resource-usage'
+
+
+def compile_library(src, out_dir=None, extra_flags=(), verbose=False,
+                    usage_out=None):
+    """hipcc ``src`` -> ``libcrs_<name>.so``; return the library path.
+    With a dict ``usage_out``, the compiler's per-kernel resource report
+    (parse_resource_usage) is stored in it."""
     out_dir = out_dir or os.path.dirname(src)
     base = os.path.splitext(os.path.basename(src))[0]
     lib = os.path.join(out_dir, 'lib{}.so'.format(base))
     cmd = [hipcc()] + HIPCC_FLAGS + list(extra_flags)
+    if usage_out is not None:
+        cmd.append(_REMARK)
     for d in INCLUDE_DIRS:
         cmd += ['-I', d]
     cmd += ['-o', lib + '.tmp', src]
@@ -45,8 +60,150 @@ def compile_library(src, out_dir=None, extra_flags=(), verbose=False):
         raise RuntimeError('hipcc failed for {}:\n{}'.format(
             src, res.stderr[-4000:]))
     os.replace(lib + '.tmp', lib)
+    if usage_out is not None:
+        usage_out.update(parse_resource_usage(res.stderr))
     if verbose:
         print('built {} in {:.1f} s'.format(lib, time.time() - t0))
+    return lib
+
+
+def parse_resource_usage(text):
+    """{kernel symbol: {field: int}} from hipcc's kernel-resource-usage
+    remarks (fields such as 'VGPRs', 'VGPRs Spill', 'ScratchSize
+    [bytes/lane]')."""
+    usage, cur = {}, None
+    for line in text.splitlines():
+        if 'remark:' not in line:
+            continue
+        body = line.split('remark:', 1)[1].split(' [-Rpass-analysis')[0]
+        body = body.strip()
+        if body.startswith('Function Name:'):
+            cur = usage.setdefault(body.split(':', 1)[1].strip(), {})
+        elif cur is not None and ':' in body:
+            key, val = body.rsplit(':', 1)
+            try:
+                cur[key.strip()] = int(val)
+            except ValueError:
+                pass
+    return usage
+
+
+# a function's entry label (local labels start with '.')
+_LABEL = re.compile(r'^([A-Za-z_][\w$]*):')
+
+
+def count_scratch_ops(asm):
+    """{kernel symbol: number of scratch load/store instructions} in gfx950
+    assembly text (for straight-line kernels, also the number each wave
+    executes)."""
+    counts, cur = {}, None
+    for line in asm.splitlines():
+        lab = _LABEL.match(line)
+        if lab:
+            cur = lab.group(1)
+            counts.setdefault(cur, 0)
+        elif cur is not None and line.lstrip().startswith('scratch_'):
+            counts[cur] += 1
+    return counts
+
+
+def device_usage(src, extra_flags=()):
+    """Resource report of ``src``'s device code alone (no library written),
+    with each kernel's scratch instruction count as 'Scratch ops'."""
+    with tempfile.TemporaryDirectory() as tmp:
+        asm = os.path.join(tmp, 'dev.s')
+        cmd = ([hipcc(), '--cuda-device-only', '-S', '-o', asm, _REMARK] +
+               [f for f in HIPCC_FLAGS if f not in ('-shared', '-fPIC')] +
+               list(extra_flags))
+        for d in INCLUDE_DIRS:
+            cmd += ['-I', d]
+        cmd.append(src)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError('hipcc failed for {}:\n{}'.format(
+                src, res.stderr[-4000:]))
+        with open(asm) as fh:
+            ops = count_scratch_ops(fh.read())
+    usage = parse_resource_usage(res.stderr)
+    for fn, u in usage.items():
+        u['Scratch ops'] = ops.get(fn, 0)
+    return usage
+
+
+# the fused kernels (emitter name -> symbol tag) and the reaction-group
+# sizes build_with_groups tries for them
+FUSED_KERNELS = (('f64', 'k_wdot_f64E'), ('mixed', 'k_wdot_mixedE'),
+                 ('dydt', 'k_dydt_f64E'))
+GROUP_CANDIDATES = (32, 40, 48, 56, 64)
+_DEFAULT_GROUP = 32
+
+
+def _fused_usage(usage):
+    """(scratch instructions, scratch bytes per lane, spilled VGPRs) of each
+    fused kernel (instructions 0 where the report has none)."""
+    out = {}
+    for short, tag in FUSED_KERNELS:
+        for fn, u in usage.items():
+            if tag in fn:
+                out[short] = (u.get('Scratch ops', 0),
+                              u.get('ScratchSize [bytes/lane]', 0),
+                              u.get('VGPRs Spill', 0))
+    return out
+
+
+def build_with_groups(write, out_dir, extra_flags=(), verbose=False,
+                      candidates=GROUP_CANDIDATES):
+    """Emit and compile a mechanism library, choosing per fused kernel the
+    number of reactions per basic block (emitter options split_f64,
+    split_mixed, split_dydt).
+
+    ``write(opts, out_dir)`` emits the source with extra emitter options
+    and returns its path.  The library is first built with 32 reactions per
+    group.  Where a fused kernel spills, the other candidate sizes are
+    compiled (device code only, in a temporary directory) and each kernel
+    takes the size with the fewest scratch instructions (in these
+    straight-line kernels also the number each wave executes), then the
+    least scratch per lane, the fewest spilled VGPRs and the size nearest
+    32: where the register allocator spills is
+    sensitive to the group boundaries, and every spilled value costs HBM
+    traffic and a reload wait (k_wdot_f64 at 53 species: 124 B/lane of
+    scratch, 56 scratch instructions and 0.650 ms at 32 reactions per
+    group; 76 B/lane, 25 instructions and 0.630 ms at 56).  The choice and every candidate's figures are written to
+    groups.json beside the library.
+    """
+    usage = {}
+    src = write({}, out_dir)
+    lib = compile_library(src, out_dir, extra_flags, verbose,
+                          usage_out=usage)
+    base = _fused_usage(usage)
+    table = {}
+    choice = {k: _DEFAULT_GROUP for k in base}
+    if any(u[1] for u in base.values()):
+        with tempfile.TemporaryDirectory() as tmp:
+            # candidates whose source is the same text (a layout whose
+            # reaction blocks do not group) are compiled once
+            srcs, texts = {}, {}
+            for c in candidates:
+                path = write({'split_blocks': c}, os.path.join(tmp, str(c)))
+                with open(path) as fh:
+                    srcs[c] = texts.setdefault(fh.read(), path)
+            with ThreadPoolExecutor(len(texts)) as pool:
+                done = dict(zip(texts.values(), pool.map(
+                    lambda p: _fused_usage(device_usage(p, extra_flags)),
+                    texts.values())))
+            table = {c: done[srcs[c]] for c in candidates}
+        for k in base:
+            choice[k] = min(table, key=lambda c: (table[c][k],
+                                                  abs(c - _DEFAULT_GROUP), c))
+        if any(v != _DEFAULT_GROUP for v in choice.values()):
+            src = write({'split_' + k: v for k, v in choice.items()},
+                        out_dir)
+            lib = compile_library(src, out_dir, extra_flags, verbose)
+    with open(os.path.join(out_dir, 'groups.json'), 'w') as fh:
+        json.dump(dict(choice=choice, candidates={
+            str(c): {k: dict(scratch_ops=v[0], scratch_bytes_per_lane=v[1],
+                             spilled_vgprs=v[2]) for k, v in u.items()}
+            for c, u in sorted(table.items())}), fh, indent=1)
     return lib
 
 
@@ -82,7 +239,9 @@ def build_mechanism(name, verbose=False, extra_flags=()):
     _e, specs, reacs, fwd = load_mechanism(
         mpath, tpath, last_spec=workloads.last_species(name))
     out_dir = os.path.join(workloads.BUILD_ROOT, name)
-    src = hip_emitter.write_hip(out_dir, specs, reacs, name=name,
-                                spec_order=fwd)
-    return compile_library(src, out_dir, extra_flags=extra_flags,
-                           verbose=verbose)
+
+    def write(opts, d):
+        return hip_emitter.write_hip(d, specs, reacs, name=name,
+                                     spec_order=fwd, opts=opts)
+    return build_with_groups(write, out_dir, extra_flags=extra_flags,
+                             verbose=verbose)

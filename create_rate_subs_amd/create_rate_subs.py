@@ -95,12 +95,13 @@ def create_rate_subs(lang, mech_name, therm_name=None, last_spec=None,
     from . import hip_emitter
     if name is None:
         name = os.path.splitext(os.path.basename(mech_name))[0]
-    src = hip_emitter.write_hip(out_dir, specs, reacs, name=name,
-                                spec_order=fwd)
+    def write(opts, d):
+        return hip_emitter.write_hip(d, specs, reacs, name=name,
+                                     spec_order=fwd, opts=opts)
+    write({}, out_dir)
     if build:
         from . import build as _build
-        lib = _build.compile_library(src, out_dir)
-        print(lib)
+        print(_build.build_with_groups(write, out_dir))
     return 0
 
 

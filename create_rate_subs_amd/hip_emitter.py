@@ -1910,7 +1910,11 @@ def _split_blocks(lines, per):
     the group.  Values a group declares for a later one (LDS reads issued
     ahead) are declared before the group and assigned inside it.  The
     exponential's constants are taken once, in registers (crs::ExpK),
-    instead of per group.
+    instead of per group.  In the concentration-row layout (c_store
+    'lane_lds', 111 species) the rows read ahead between reactions end a
+    group, so each reaction is its own basic block there: measured, that
+    spills least (207 scratch instructions in k_wdot_f64 at 111 species
+    against 394-549 with 32-64 reactions per group).
     """
     lines = ['const crs::ExpK ek = crs::exp_consts();'] + [
         l.replace('crs::exp2_64(', 'crs::exp2_64k(ek, ')
@@ -2225,6 +2229,10 @@ def emit_hip_source(ir, opts=None):
         table (declared here, filled by the kernel)."""
         mode = opts.get('consts_rx')
         nfill = nasa_fill if o.get('nasa') == 'lds' else ''
+        # reactions per basic block, per kernel (split_f64, split_mixed,
+        # split_dydt) over the common split_blocks
+        if 'split_' + name in o:
+            o = dict(o, split_blocks=o['split_' + name])
         if mode not in ('lds', 'lds_fp64') or (mode == 'lds_fp64' and
                                                 name == 'mixed'):
             return _fused_fast_lines(ir, K, o, **kw), nfill

@@ -49,21 +49,40 @@ def variant_dir(mech, name):
     return os.path.join(workloads.BUILD_ROOT, 'variants', mech, name)
 
 
+def default_groups(mech):
+    """Reactions per basic block the default library of ``mech`` was built
+    with (build.build_with_groups' choice), as emitter options."""
+    path = os.path.join(os.path.dirname(build.lib_path(mech)), 'groups.json')
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        choice = json.load(fh).get('choice', {})
+    return {'split_' + k: v for k, v in choice.items()}
+
+
 def build_variant(mech, name, opts):
     """Emit and compile mechanism ``mech`` with emitter options ``opts``
-    (the key 'hipcc_flags', a list, is passed to hipcc instead)."""
+    (the key 'hipcc_flags', a list, is passed to hipcc instead).  Unless
+    ``opts`` sets the group size, the variant keeps the default library's
+    reactions per basic block, so that it differs from the default only in
+    what ``opts`` names (build the default library first)."""
     from .create_rate_subs import load_mechanism
     mpath, tpath = workloads.mechanism_files(mech)
     _e, specs, reacs, fwd = load_mechanism(mpath, tpath)
     out = variant_dir(mech, name)
     opts = dict(opts)
     flags = opts.pop('hipcc_flags', [])
+    groups = ({} if any(k.startswith('split_') for k in opts) else
+              default_groups(mech))
+    full = dict(groups, **opts)
     src = hip_emitter.write_hip(out, specs, reacs, name=mech, spec_order=fwd,
-                                opts=opts)
+                                opts=full)
     lib = build.compile_library(src, out, extra_flags=flags)
+    # 'opts' are the variant's own options, 'groups' the group sizes taken
+    # over from the default library
     with open(os.path.join(out, 'opts.json'), 'w') as fh:
-        json.dump(dict(opts=opts, hipcc_flags=flags,
-                       resolved=hip_emitter.resolve_opts(len(specs), opts)),
+        json.dump(dict(opts=opts, groups=groups, hipcc_flags=flags,
+                       resolved=hip_emitter.resolve_opts(len(specs), full)),
                   fh, indent=1)
     return lib
 
