@@ -7,6 +7,7 @@ compiles one of the synthetic mechanisms into ``_build/<name>/``;
 register allocator spills least at (see its docstring).
 """
 
+import hashlib
 import json
 import os
 import re
@@ -134,7 +135,7 @@ def device_usage(src, extra_flags=()):
 # sizes build_with_groups tries for them
 FUSED_KERNELS = (('f64', 'k_wdot_f64E'), ('mixed', 'k_wdot_mixedE'),
                  ('dydt', 'k_dydt_f64E'))
-GROUP_CANDIDATES = (32, 40, 48, 56, 64)
+GROUP_CANDIDATES = (32,) + tuple(range(36, 61)) + (64,)
 _DEFAULT_GROUP = 32
 
 
@@ -151,6 +152,20 @@ def _fused_usage(usage):
     return out
 
 
+def _groups_key(src_text, extra_flags, candidates):
+    """Hash of everything the group-size choice depends on: the emitted
+    source, the device headers it includes, the flags and the candidates."""
+    h = hashlib.sha256(src_text.encode())
+    for d in INCLUDE_DIRS:
+        for f in sorted(os.listdir(d)):
+            if f.endswith('.h'):
+                with open(os.path.join(d, f), 'rb') as fh:
+                    h.update(fh.read())
+    h.update(repr((HIPCC_FLAGS, list(extra_flags), list(candidates)))
+             .encode())
+    return h.hexdigest()
+
+
 def build_with_groups(write, out_dir, extra_flags=(), verbose=False,
                       candidates=GROUP_CANDIDATES):
     """Emit and compile a mechanism library, choosing per fused kernel the
@@ -158,21 +173,37 @@ def build_with_groups(write, out_dir, extra_flags=(), verbose=False,
     split_mixed, split_dydt).
 
     ``write(opts, out_dir)`` emits the source with extra emitter options
-    and returns its path.  The library is first built with 32 reactions per
-    group.  Where a fused kernel spills, the other candidate sizes are
-    compiled (device code only, in a temporary directory) and each kernel
-    takes the size with the fewest scratch instructions (in these
-    straight-line kernels also the number each wave executes), then the
-    least scratch per lane, the fewest spilled VGPRs and the size nearest
-    32: where the register allocator spills is
-    sensitive to the group boundaries, and every spilled value costs HBM
-    traffic and a reload wait (k_wdot_f64 at 53 species: 124 B/lane of
-    scratch, 56 scratch instructions and 0.650 ms at 32 reactions per
-    group; 76 B/lane, 25 instructions and 0.630 ms at 56).  The choice and every candidate's figures are written to
-    groups.json beside the library.
+    and returns its path.  The library is first built with 32 reactions
+    per group.  Where a fused kernel spills, every candidate size (32,
+    36-60, 64) is compiled (device code only, in a temporary directory) and
+    each kernel takes the size with the fewest scratch instructions (in
+    these straight-line kernels also the number each wave executes), then
+    the least scratch per lane, the fewest spilled VGPRs and the size
+    nearest 32.  Where the register allocator spills depends on the group
+    boundaries, and every spilled value costs HBM traffic and a reload
+    wait: k_wdot_f64 at 53 species has 124 B/lane of scratch, 56 scratch
+    instructions and runs 0.650 ms at 32 reactions per group; 76 B/lane,
+    25 instructions and 0.630 ms at 56; 44 B/lane and 15 instructions at
+    47 (the count does not vary smoothly with the size).  The choice, every
+    candidate's figures and a hash of the inputs are written to groups.json
+    beside the library; a later build of the same source, headers and
+    flags takes the recorded choice instead of compiling the candidates
+    again (the library itself is always compiled).
     """
     usage = {}
     src = write({}, out_dir)
+    with open(src) as fh:
+        key = _groups_key(fh.read(), extra_flags, candidates)
+    record = os.path.join(out_dir, 'groups.json')
+    if os.path.exists(record):
+        with open(record) as fh:
+            old = json.load(fh)
+        if old.get('key') == key:
+            choice = old['choice']
+            if any(v != _DEFAULT_GROUP for v in choice.values()):
+                src = write({'split_' + k: v for k, v in choice.items()},
+                            out_dir)
+            return compile_library(src, out_dir, extra_flags, verbose)
     lib = compile_library(src, out_dir, extra_flags, verbose,
                           usage_out=usage)
     base = _fused_usage(usage)
@@ -187,7 +218,8 @@ def build_with_groups(write, out_dir, extra_flags=(), verbose=False,
                 path = write({'split_blocks': c}, os.path.join(tmp, str(c)))
                 with open(path) as fh:
                     srcs[c] = texts.setdefault(fh.read(), path)
-            with ThreadPoolExecutor(len(texts)) as pool:
+            with ThreadPoolExecutor(min(len(texts),
+                                        os.cpu_count() or 4)) as pool:
                 done = dict(zip(texts.values(), pool.map(
                     lambda p: _fused_usage(device_usage(p, extra_flags)),
                     texts.values())))
@@ -199,8 +231,8 @@ def build_with_groups(write, out_dir, extra_flags=(), verbose=False,
             src = write({'split_' + k: v for k, v in choice.items()},
                         out_dir)
             lib = compile_library(src, out_dir, extra_flags, verbose)
-    with open(os.path.join(out_dir, 'groups.json'), 'w') as fh:
-        json.dump(dict(choice=choice, candidates={
+    with open(record, 'w') as fh:
+        json.dump(dict(key=key, choice=choice, candidates={
             str(c): {k: dict(scratch_ops=v[0], scratch_bytes_per_lane=v[1],
                              spilled_vgprs=v[2]) for k, v in u.items()}
             for c, u in sorted(table.items())}), fh, indent=1)
