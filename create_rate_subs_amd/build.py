@@ -261,6 +261,24 @@ def build_mlp(verbose=False):
     return lib
 
 
+PROBE_SRC = os.path.join(HERE, 'csrc', 'fma_probe.hip')
+PROBE_LIB = os.path.join(workloads.BUILD_ROOT, 'libcrs_probe.so')
+
+
+def build_probe(verbose=False):
+    """Compile the FP64 FMA-ceiling probe (tune.ceiling)."""
+    os.makedirs(workloads.BUILD_ROOT, exist_ok=True)
+    cmd = [hipcc()] + HIPCC_FLAGS + ['-o', PROBE_LIB + '.tmp', PROBE_SRC]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('hipcc failed for fma_probe.hip:\n' +
+                           res.stderr[-4000:])
+    os.replace(PROBE_LIB + '.tmp', PROBE_LIB)
+    if verbose:
+        print('built', PROBE_LIB)
+    return PROBE_LIB
+
+
 def lib_path(name):
     return os.path.join(workloads.BUILD_ROOT, name, 'libcrs_{}.so'.format(name))
 

@@ -13,6 +13,8 @@ conditioned error of create_rate_subs_amd.metrics:
     python -m create_rate_subs_amd.tune run mech53 b128
     # on the GPU: the default library against the batch size
     python -m create_rate_subs_amd.tune sweep mech53
+    # on the GPU: plain FP64 FMA chains, the VALU issue ceiling
+    python -m create_rate_subs_amd.tune ceiling
 
 Variants live under _build/variants/<mechanism>/<name>/ with the options
 they were built from (opts.json).  One JSON line per (variant, mode).
@@ -198,6 +200,45 @@ def sweep(mech, exps=range(10, 23, 2), steps=10, seed=2001, rounds=3):
                                   blocks=(n + 255) // 256)), flush=True)
 
 
+def ceiling(ks=(1, 2, 4, 8), waves=(1, 2, 4), blocks=256, n=8192,
+            launches=5):
+    """FP64 FMA rate of plain FMA chains (csrc/fma_probe.hip) at 1, 2 and
+    4 waves per SIMD and 1-8 independent chains per lane, one block per CU:
+    the VALU issue ceiling the fused kernels' VALU share is read against.
+    One JSON line per (waves, chains)."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(build.PROBE_LIB)
+    lib.crs_probe_fma.restype = ctypes.c_int
+    lib.crs_probe_fma.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2
+    stream = torch.cuda.current_stream()
+    for w in waves:
+        out = torch.empty(blocks * 256 * w, dtype=torch.float64,
+                          device='cuda')
+        for k in ks:
+            m = max(1, n // k)
+
+            def launch():
+                rc = lib.crs_probe_fma(k, w, blocks, m, out.data_ptr(),
+                                       stream.cuda_stream)
+                if rc:
+                    raise RuntimeError('crs_probe_fma failed')
+            launch()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(launches):
+                launch()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / launches
+            flops = 2.0 * blocks * 256 * w * m * 16 * k
+            print(json.dumps(dict(waves_per_simd=w, chains=k, ms=ms,
+                                  tflops=flops / (ms * 1e-3) / 1e12,
+                                  frac_of_78p6=flops / (ms * 1e-3) / 78.6e12)),
+                  flush=True)
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     if len(argv) >= 4 and argv[0] == 'build':
@@ -206,6 +247,8 @@ def main(argv=None):
         run(argv[1], argv[2:])
     elif len(argv) >= 2 and argv[0] == 'sweep':
         sweep(argv[1])
+    elif argv and argv[0] == 'ceiling':
+        ceiling()
     else:
         print(__doc__)
         return 2
