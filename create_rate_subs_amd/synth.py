@@ -61,6 +61,18 @@ class MechConfig:
     # with explicit reverse parameters (REV), every 3rd irreversible
     # elementary one with all coefficients halved (non-integer orders)
     form_variants: bool = False
+    # thermochemistry and rate-constant draws (the defaults are the recipe
+    # of the workloads; narrower ones give a non-stiff mechanism): h298 ~
+    # N(0, h298_sd) J/mol, s298 ~ U(s298_range) J/mol/K, log10 A ~
+    # U(logA_range) + logA_per_order (order - 1) in cgs units, with order
+    # the reactant coefficient sum (+1 for a third body), b ~ U(b_range),
+    # Ea ~ U(Ea_range) kcal/mol
+    h298_sd: float = 150.0e3
+    s298_range: tuple = (150.0, 300.0)
+    logA_range: tuple = (8.0, 16.0)
+    logA_per_order: float = 0.0
+    b_range: tuple = (-2.0, 3.0)
+    Ea_range: tuple = (0.0, 60.0)
 
 
 # The three mechanisms the workloads are built on.
@@ -81,6 +93,15 @@ MECHS = {
     'mech19fx': MechConfig('mech19fx', 19, 80, 0.80, 0.10, 3, frac_troe=0.10,
                            frac_lind=0.10, frac_sri=0.08, seed=1009,
                            form_variants=True),
+    # non-stiff: Gibbs energies within a few kJ/mol of each other (moderate
+    # equilibrium constants), pre-exponential factors scaled by reaction
+    # order and activation energies of 18-24 kcal/mol, so that every time
+    # scale at 1200-1500 K and 1 atm is longer than about 0.1 ms: an
+    # explicit integrator converges at steps of microseconds
+    'mech9ns': MechConfig('mech9ns', 9, 14, 0.80, 0.20, 2, seed=1011,
+                          h298_sd=4.0e3, s298_range=(195.0, 225.0),
+                          logA_range=(5.0, 6.0), logA_per_order=5.5,
+                          b_range=(0.0, 0.5), Ea_range=(22.0, 28.0)),
 }
 
 # PLOG table pressures [atm] and Chebyshev validity box of the generator
@@ -108,13 +129,13 @@ def _pick_compositions(rng, count):
     return lattice[:count]
 
 
-def _fit_nasa(rng, tm=1000.0):
+def _fit_nasa(rng, tm=1000.0, h_sd=150.0e3, s_range=(150.0, 300.0)):
     """Random smooth cp/R -> NASA-7 (lo, hi) coefficient lists, ranges
     [300, tm] and [tm, 5000] K."""
     c = rng.uniform(0.5, 8.0)
     theta = rng.uniform(500.0, 3000.0)
-    h298 = rng.normal(0.0, 150.0e3)           # J/mol
-    s298 = rng.uniform(150.0, 300.0)          # J/mol/K
+    h298 = rng.normal(0.0, h_sd)              # J/mol
+    s298 = rng.uniform(*s_range)              # J/mol/K
     R = 8.3144621
 
     def cp(T):
@@ -197,11 +218,17 @@ def _side_text(terms, names):
     return ' + '.join(out)
 
 
-def _arrhenius(rng):
-    logA = rng.uniform(8.0, 16.0)
+def _arrhenius(rng, cfg=None, order=1.0):
+    """(A, b, Ea) drawn as MechConfig describes (the workloads' ranges
+    without ``cfg``)."""
+    la, br, er = ((cfg.logA_range, cfg.b_range, cfg.Ea_range) if cfg else
+                  ((8.0, 16.0), (-2.0, 3.0), (0.0, 60.0)))
+    logA = rng.uniform(*la)
+    if cfg is not None:
+        logA += cfg.logA_per_order * (order - 1.0)
     A = float('%.6E' % (10.0 ** logA))
-    b = round(rng.uniform(-2.0, 3.0), 3)
-    Ea = round(rng.uniform(0.0, 60.0) * 1000.0, 3)    # cal/mol
+    b = round(rng.uniform(*br), 3)
+    Ea = round(rng.uniform(*er) * 1000.0, 3)    # cal/mol
     return A, b, Ea
 
 
@@ -310,7 +337,8 @@ def generate_mechanism(cfg):
     rxns = []
     for j, (reac, prod) in enumerate(balanced):
         r = _Rxn(reac=reac, prod=prod, rev=bool(rev_flags[j]), kind=kinds[j])
-        r.arr = _arrhenius(rng)
+        r.arr = _arrhenius(rng, cfg, sum(nu for _k, nu in reac) +
+                           (1 if kinds[j] == 'thd' else 0))
         if r.kind == 'plog':
             # ln k varies smoothly with ln P: a shifted A per pressure
             A, b, Ea = r.arr
@@ -357,12 +385,13 @@ def generate_mechanism(cfg):
     thermo = ['THERMO\n', '   300.000  1000.000  5000.000\n']
     for i, comp in enumerate(comps):
         tm = cfg.tmid_alt if (cfg.tmid_alt and i % 3 == 1) else 1000.0
-        lo, hi = _fit_nasa(rng, tm)
+        lo, hi = _fit_nasa(rng, tm, cfg.h298_sd, cfg.s298_range)
         counts = [(el, n) for el, n in zip(_ELEMS, comp) if n]
         thermo.append(_thermo_entry(names[i], counts, lo, hi, tm))
     # bath gas: drawn like the others from its own stream; it only ever
     # acts as a collider, so its thermo never enters an equilibrium constant
-    lo, hi = _fit_nasa(np.random.default_rng(cfg.seed + 7919))
+    lo, hi = _fit_nasa(np.random.default_rng(cfg.seed + 7919), 1000.0,
+                       cfg.h298_sd, cfg.s298_range)
     thermo.append(_thermo_entry('N2', [('N', 2)], lo, hi))
     thermo.append('END\n')
 

@@ -21,7 +21,7 @@ from _helpers import hip_lib, mechanism, oracle, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 MECHS = ['mech13', 'mech53', 'mech111', 'mech13co', 'mech21pc',
-         'mech17tm', 'mech19fx']
+         'mech17tm', 'mech19fx', 'mech9ns']
 # tau of the conditioned error: where |wdot_k| < tau * S_k, errors count
 # relative to tau * S_k.  Every rate constant is exp(x) with |x| up to
 # ~100 (ln A, E/T and the NASA polynomial of ln Kc), and two correct FP64
