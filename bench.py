@@ -202,7 +202,7 @@ def dydt_section(args, torch, lib, T, P, Y):
     return dict(workload='w1_53 states in the reference dydt layout '
                          '(row 0 = T, rows 1..NSP-1 = Y)',
                 states_per_s=n / (ms * 1e-3), kernel_ms=kern,
-                kernel='k_dydt_f64')
+                kernel='k_wdot_f64 + k_dydt_energy')
 
 
 def integrate_section(args, torch, lib, specs, n, steps=10):
@@ -226,7 +226,7 @@ def integrate_section(args, torch, lib, specs, n, steps=10):
     out = integrate.integrate(lib, P, y, 0.0, steps * h, h)
     finite = float(torch.isfinite(y).all(0).double().mean())
     return dict(workload='{} near-equilibrium states of the w1_53 '
-                         'mechanism, {} RK4 steps (4 dydt + 4 stage kernels '
+                         'mechanism, {} RK4 steps (4 wdot + 4 energy-and-stage kernels '
                          'each) from one graph'.format(n, out['steps']),
                 steps=out['steps'], h=h, ms_per_step=out['ms_per_step'],
                 state_steps_per_s=n * out['steps'] / (out['ms_total'] * 1e-3),

@@ -134,7 +134,7 @@ def device_usage(src, extra_flags=()):
 # the fused kernels (emitter name -> symbol tag) and the reaction-group
 # sizes build_with_groups tries for them
 FUSED_KERNELS = (('f64', 'k_wdot_f64E'), ('mixed', 'k_wdot_mixedE'),
-                 ('dydt', 'k_dydt_f64E'))
+                 ('dydt', 'k_dydt_f64E'))   # the last with dydt='fused'
 GROUP_CANDIDATES = (32,) + tuple(range(36, 61)) + (64,)
 _DEFAULT_GROUP = 32
 

@@ -1573,10 +1573,11 @@ def _dydt_cp_lines(ir, K, tmids):
     return out
 
 
-def _dydt_finish_lines(ir, K, tmids, k, wk):
+def _dydt_finish_lines(ir, K, tmids, k, wk, store='W[{r} * ld + i] = {v};'):
     """Species k's rate wk is final: its term of the energy equation
     (w_k h_k W_k, h_k = RU/W_k (a5 + a0 T + a1 T^2/2 + ... + a4 T^5/5),
-    rate_subs.py:1495-1565) and, for k < NSP-1, dY_k/dt = w_k W_k / rho."""
+    rate_subs.py:1495-1565) and, for k < NSP-1, dY_k/dt = w_k W_k / rho,
+    written as ``store`` formats it (row r, value v)."""
     poly = _nasa_masked(
         ir, K, k, tmids,
         lambda a: [a[5], a[0], a[1] / 2.0, a[2] / 3.0, a[3] / 4.0, a[4] / 5.0],
@@ -1584,9 +1585,89 @@ def _dydt_finish_lines(ir, K, tmids, k, wk):
     out = ['esum = esum + ({w}) * ({ru} * ({p})) * {mw};'.format(
         w=wk, ru=K(ir.ru_mw[k]), p=poly, mw=K(ir.mw[k]))]
     if k < ir.nsp - 1:
-        out.append('W[{r} * ld + i] = ({w}) * ({mw} * irho);'.format(
-            r=k + 1, w=wk, mw=K(ir.mw[k])))
+        out.append(store.format(r=k + 1, v='({w}) * ({mw} * irho)'.format(
+            w=wk, mw=K(ir.mw[k]))))
     return out
+
+
+def _dydt_energy_lines(ir, K, chunk=8):
+    """Body of the energy stage of the split CONP dydt (rate_subs.py:
+    1829-1946 after eval_spec_rates): from the state (Y: row 0 = T, rows
+    1..NSP-1 the mass fractions), P and the species rates W (rows
+    0..NSP-1, from the wdot kernel), dT/dt and dY_k/dt through the output
+    object: ``out.fetch(r)`` for the rows of a chunk of ``chunk`` species,
+    then ``out.put(r, value)`` for each (rows 1..NSP-1, row 0 last).
+
+    Two passes keep the live set small (the kernel is memory bound and wants
+    occupancy): the mass fractions are summed as they load (y_N, mw_avg,
+    cp_avg, in the reference's order), then the rates are taken in chunks
+    from the last species down, every rate of a chunk loaded before the
+    chunk's outputs: rows are written only after they have been read, so W
+    may be the output array itself."""
+    nsp = ir.nsp
+    tmids = []
+    for tm, _lo, _hi in ir.nasa:
+        if tm not in tmids:
+            tmids.append(tm)
+    ru = _lit(emitted(fmt_e8(chem.RU)))
+    # rows are reached through a uniform row base and the lane's 32-bit
+    # byte offset io (crs::ld_row): no 64-bit address per row to keep live
+    lines = ['const uint32_t io = (uint32_t)i * 8u;',
+             'const double T = crs::ld_row(Y, io);',
+             'const double P = crs::ld_row(P_, io);',
+             'const double T2 = T * T;', 'const double T3 = T2 * T;',
+             'const double T4 = T3 * T;', 'const double T5 = T4 * T;']
+    vals = {_B1: '1.0', _BT: 'T', _BT2: 'T2', _BT3: 'T3', _BT4: 'T4',
+            _BT5: 'T5'}
+    for gi, tm in enumerate(tmids):
+        lines.append('const bool lo{g} = T <= {tm};'.format(g=gi,
+                                                           tm=_lit(tm)))
+        for bi, v in sorted(vals.items()):
+            lines.append('const double pl{g}_{b} = lo{g} ? {v} : 0.0;'.format(
+                g=gi, b=bi, v=v))
+            lines.append('const double ph{g}_{b} = lo{g} ? 0.0 : {v};'.format(
+                g=gi, b=bi, v=v))
+    # pass 1: sums over the mass fractions, each term in the reference's
+    # order (eval_conc's y_N and mw_avg, then the cp_avg sum of dydt)
+    lines += ['double ysum = 0.0;', 'double mwsum = 0.0;',
+              'double cp_avg = 0.0;']
+    cps = []
+    for k in range(nsp):
+        cps.append(_nasa_masked(ir, K, k, tmids, lambda a: a[0:5],
+                                (_B1, _BT, _BT2, _BT3, _BT4)))
+    for k in range(nsp - 1):
+        lines.append('{{ const double yk = crs::ld_row(Y + {r} * ld, io);'
+                     .format(r=k + 1))
+        lines.append('  ysum = {}yk;'.format('ysum + ' if k else ''))
+        lines.append('  mwsum = {}(yk * {});'.format('mwsum + ' if k else '',
+                                                      K(ir.inv_mw[k])))
+        lines.append('  cp_avg = cp_avg + {} * ({}) * yk; }}'.format(
+            K(ir.ru_mw[k]), cps[k]))
+    lines += ['const double y_N = 1.0 - ysum;',
+              'const double mw_avg = 1.0 / (mwsum + (y_N * {}));'.format(
+                  K(ir.inv_mw[nsp - 1])),
+              'const double rho = P * mw_avg / ({} * T);'.format(ru),
+              'cp_avg = cp_avg + {} * ({}) * y_N;'.format(
+                  K(ir.ru_mw[nsp - 1]), cps[nsp - 1]),
+              'const double irho = 1.0 / rho;', 'double esum = 0.0;']
+    # pass 2: the rates in chunks from the last species down
+    starts = list(range(0, nsp, chunk))[::-1]
+    for k0 in starts:
+        ks = list(range(k0, min(k0 + chunk, nsp)))
+        lines.append('{')
+        lines += ['  const double w{k} = crs::ld_row(W + {k} * ld, io);'
+                  .format(k=k) for k in ks]
+        lines += ['  out.fetch({});'.format(k + 1) for k in ks
+                  if k < nsp - 1]
+        if k0 == 0:
+            lines.append('  out.fetch(0);')
+        for k in ks[::-1]:
+            lines += ['  ' + l for l in _dydt_finish_lines(
+                ir, K, tmids, k, 'w{}'.format(k), store='out.put({r}, {v});')]
+        lines.append('}')
+    # energy equation, rate_subs.py:1910-1925
+    lines.append('out.put(0, (-1.0 / (rho * cp_avg)) * esum);')
+    return lines
 
 
 def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
@@ -2084,6 +2165,14 @@ def emit_hip_source(ir, opts=None):
     # its cp and h terms add live registers that two waves would spill
     dopts = (resolve_opts(ir.nsp, dict(raw, waves_per_simd=None))
              if raw.get('waves_per_simd', 'auto') == 'auto' else opts)
+    # dydt='split' (default): crs_dydt runs the wdot kernel and a light
+    # energy-equation kernel (k_dydt_energy), crs_rk4 the wdot kernel and
+    # an energy + RK4-stage kernel; dydt='fused': one kernel, k_dydt_f64
+    dydt_split = opts.get('dydt', 'split') == 'split'
+    if opts.get('dydt', 'split') not in ('split', 'fused'):
+        raise ValueError("dydt must be 'split' or 'fused'")
+    if dydt_split:
+        dopts = opts
     K = _Consts(opts['consts'])
     nsp, nr, nrev, npd = ir.nsp, ir.nr, ir.nrev, ir.npd
     ru = _lit(emitted(fmt_e8(chem.RU)))
@@ -2282,77 +2371,177 @@ def emit_hip_source(ir, opts=None):
                          preload))
 
     # ---- constant-pressure dydt (FP64), the caller of the path -------
-    dydt_lines, dydt_fill = fused_state(
-        'dydt', dict(dopts, preload=preload), dydt=True)
-    body.append(
-        '__device__ __forceinline__ void dydt_f64_state(\n'
-        '    const double *__restrict__ K, const double *__restrict__ P_,\n'
-        '    const double *__restrict__ Y, double *__restrict__ W,\n'
-        '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
-        _indent(dydt_lines, 4) + '}\n\n')
-    if preload and dydt_fill:
-        dydt_tail = (
-            '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
-            'threadIdx.x;\n'
-            '    const bool live = i < n;\n'
-            '    const double T_pre = live ? Y[i] : 0.0;\n'
-            '    const double P_pre = live ? P_[i] : 0.0;\n' + dydt_fill +
-            '    if (!live) return;\n'
-            '    dydt_f64_state(K, P_, Y, W, ld, i, T_pre, P_pre);\n}\n\n')
-    else:
-        dydt_tail = (
-            dydt_fill +
+    if dydt_split:
+        body.append(
+            '// Energy stage of the CONP dydt: the state (row 0 = T, rows\n'
+            '// 1..NSP-1 the mass fractions), P and the species rates W of\n'
+            '// the wdot kernel give dT/dt and dY_k/dt, handed to the output\n'
+            '// object (fetch(row) ahead of a chunk, put(row, value)).  No\n'
+            '// contraction: the values are the same bits in k_dydt_energy\n'
+            '// and in k_rk4_stage.\n'
+            'template <class Out>\n'
+            '__device__ __forceinline__ void dydt_energy_state(\n'
+            '    const double *__restrict__ P_, const double *Y,\n'
+            '    const double *W, int64_t ld, int64_t i, Out &out)\n{\n'
+            '#pragma clang fp contract(off)\n' +
+            _indent(_dydt_energy_lines(ir, K), 4) + '}\n\n')
+        body.append(
+            '// crs_dydt\'s output: the rates W were written into dy itself by\n'
+            '// the wdot kernel, and each row is overwritten only after it\n'
+            '// has been read.\n'
+            'struct DydtOut {\n'
+            '    double *dy;\n'
+            '    int64_t ld;\n'
+            '    uint32_t io;\n'
+            '    __device__ __forceinline__ void fetch(int) {}\n'
+            '    __device__ __forceinline__ void put(int r, double v)\n'
+            '    {\n'
+            '        crs::st_row(dy + r * ld, io, v);\n'
+            '    }\n'
+            '};\n\n'
+            '__global__ __launch_bounds__(crs::kBlock) void k_dydt_energy(\n'
+            '    const double *__restrict__ P_, const double *__restrict__ Y,\n'
+            '    double *W, int64_t n, int64_t ld)\n{\n'
             '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
             'threadIdx.x;\n'
             '    if (i >= n) return;\n'
-            '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
-    body.append(
-        '__global__ __launch_bounds__(crs::kBlock' + dlb + ') void k_dydt_f64(\n'
-        '    const double *__restrict__ K, const double *__restrict__ P_,\n'
-        '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
-        '    int64_t ld)\n{\n' + dydt_tail)
+            '    DydtOut out{W, ld, (uint32_t)i * 8u};\n'
+            '    dydt_energy_state(P_, Y, W, ld, i, out);\n}\n\n')
+        body.append(
+            '// One RK4 stage: k = dydt(ys) from the rates the wdot kernel\n'
+            '// wrote for the stage state ys (the energy stage in registers),\n'
+            '// then, without contraction, the textbook combination:\n'
+            '//   1: acc = k,      ytmp = y + h/2 k\n'
+            '//   2: acc += 2 k,   ytmp = y + h/2 k\n'
+            '//   3: acc += 2 k,   ytmp = y + h k\n'
+            '//   4: y += h/6 (acc + k); a state whose T first reaches\n'
+            '//      T0 + dT_ign gets t_ign = t_step, the time at the start of\n'
+            '//      this step, as the reference records t before its t += h\n'
+            '//      (create_rate_subs.py:267-276).  NaN marks a state that has\n'
+            '//      not ignited yet, so any finite time (a negative t0 too) is\n'
+            '//      kept once set.\n'
+            '// fetch(row) loads the row\'s y and acc ahead of the chunk\'s\n'
+            '// stores.  No pointer is restrict-qualified on purpose: the\n'
+            '// possible aliasing keeps each chunk\'s loads behind the\n'
+            '// previous chunk\'s stores, so the compiler cannot hoist every\n'
+            '// row of the state into registers (it would spill).\n'
+            'template <int STAGE>\n'
+            'struct Rk4Out {\n'
+            '    double *y, *ytmp, *acc;\n'
+            '    int64_t ld;\n'
+            '    uint32_t io;\n'
+            '    double h;\n'
+            '    double yv[NSP], av[NSP];\n'
+            '    __device__ __forceinline__ void fetch(int r)\n'
+            '    {\n'
+            '        yv[r] = crs::ld_row(y + r * ld, io);\n'
+            '        av[r] = STAGE > 1 ? crs::ld_row(acc + r * ld, io) : 0.0;\n'
+            '    }\n'
+            '    __device__ __forceinline__ void put(int r, double kv)\n'
+            '    {\n'
+            '#pragma clang fp contract(off)\n'
+            '        if (STAGE == 1) {\n'
+            '            crs::st_row(acc + r * ld, io, kv);\n'
+            '            crs::st_row(ytmp + r * ld, io, yv[r] + 0.5 * h * kv);\n'
+            '        } else if (STAGE == 2) {\n'
+            '            crs::st_row(acc + r * ld, io, av[r] + 2.0 * kv);\n'
+            '            crs::st_row(ytmp + r * ld, io, yv[r] + 0.5 * h * kv);\n'
+            '        } else if (STAGE == 3) {\n'
+            '            crs::st_row(acc + r * ld, io, av[r] + 2.0 * kv);\n'
+            '            crs::st_row(ytmp + r * ld, io, yv[r] + h * kv);\n'
+            '        } else {\n'
+            '            crs::st_row(y + r * ld, io,\n'
+            '                        yv[r] + (h / 6.0) * (av[r] + kv));\n'
+            '        }\n'
+            '    }\n'
+            '};\n\n'
+            '// (at least four waves per SIMD: the kernel is memory bound)\n'
+            'template <int STAGE>\n'
+            '__global__ __launch_bounds__(crs::kBlock, 4) void k_rk4_stage(\n'
+            '    const double *__restrict__ P_, const double *wk, double *y,\n'
+            '    double *ytmp, double *acc, int64_t n, int64_t ld, double h,\n'
+            '    double t_step, const double *__restrict__ T0,\n'
+            '    double *__restrict__ t_ign, double dT_ign)\n{\n'
+            '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
+            'threadIdx.x;\n'
+            '    if (i >= n) return;\n'
+            '    Rk4Out<STAGE> out{y, ytmp, acc, ld, (uint32_t)i * 8u, h, {}, {}};\n'
+            '    dydt_energy_state(P_, STAGE == 1 ? y : ytmp, wk, ld, i, out);\n'
+            '    if (STAGE == 4 && t_ign != nullptr && t_ign[i] != t_ign[i] &&\n'
+            '        y[i] >= T0[i] + dT_ign)\n'
+            '        t_ign[i] = t_step;\n}\n\n')
+    else:
+        dydt_lines, dydt_fill = fused_state(
+            'dydt', dict(dopts, preload=preload), dydt=True)
+        body.append(
+            '__device__ __forceinline__ void dydt_f64_state(\n'
+            '    const double *__restrict__ K, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Y, double *__restrict__ W,\n'
+            '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
+            _indent(dydt_lines, 4) + '}\n\n')
+        if preload and dydt_fill:
+            dydt_tail = (
+                '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
+                'threadIdx.x;\n'
+                '    const bool live = i < n;\n'
+                '    const double T_pre = live ? Y[i] : 0.0;\n'
+                '    const double P_pre = live ? P_[i] : 0.0;\n' + dydt_fill +
+                '    if (!live) return;\n'
+                '    dydt_f64_state(K, P_, Y, W, ld, i, T_pre, P_pre);\n}\n\n')
+        else:
+            dydt_tail = (
+                dydt_fill +
+                '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
+                'threadIdx.x;\n'
+                '    if (i >= n) return;\n'
+                '    dydt_f64_state(K, P_, Y, W, ld, i);\n}\n\n')
+        body.append(
+            '__global__ __launch_bounds__(crs::kBlock' + dlb + ') void k_dydt_f64(\n'
+            '    const double *__restrict__ K, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Y, double *__restrict__ W, int64_t n,\n'
+            '    int64_t ld)\n{\n' + dydt_tail)
 
-    # ---- RK4 stage combination for the batched integrator ------------
-    body.append(
-        '// One RK4 stage on the state rows 0..NSP-1 (row 0 = T), written\n'
-        '// without contraction so that it is the textbook sequence exactly:\n'
-        '//   1: acc = k,      ytmp = y + h/2 k\n'
-        '//   2: acc += 2 k,   ytmp = y + h/2 k\n'
-        '//   3: acc += 2 k,   ytmp = y + h k\n'
-        '//   4: y += h/6 (acc + k); a state whose T first reaches\n'
-        '//      T0 + dT_ign gets t_ign = t_step, the time at the start of\n'
-        '//      this step, as the reference records t before its t += h\n'
-        '//      (create_rate_subs.py:267-276).  NaN marks a state that has\n'
-        '//      not ignited yet, so any finite time (a negative t0 too) is\n'
-        '//      kept once set.\n'
-        '__global__ __launch_bounds__(crs::kBlock) void k_rk4_combine(\n'
-        '    double *__restrict__ y, double *__restrict__ ytmp,\n'
-        '    double *__restrict__ acc, const double *__restrict__ k,\n'
-        '    int64_t n, int64_t ld, int stage, double h, double t_step,\n'
-        '    const double *__restrict__ T0, double *__restrict__ t_ign,\n'
-        '    double dT_ign)\n{\n'
-        '#pragma clang fp contract(off)\n'
-        '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
-        '    if (i >= n) return;\n'
-        '    for (int r = 0; r < NSP; ++r) {\n'
-        '        const int64_t o = (int64_t)r * ld + i;\n'
-        '        const double kv = k[o];\n'
-        '        if (stage == 1) {\n'
-        '            acc[o] = kv;\n'
-        '            ytmp[o] = y[o] + 0.5 * h * kv;\n'
-        '        } else if (stage == 2) {\n'
-        '            acc[o] += 2.0 * kv;\n'
-        '            ytmp[o] = y[o] + 0.5 * h * kv;\n'
-        '        } else if (stage == 3) {\n'
-        '            acc[o] += 2.0 * kv;\n'
-        '            ytmp[o] = y[o] + h * kv;\n'
-        '        } else {\n'
-        '            y[o] += (h / 6.0) * (acc[o] + kv);\n'
-        '        }\n'
-        '    }\n'
-        '    if (stage == 4 && t_ign != nullptr && t_ign[i] != t_ign[i] &&\n'
-        '        y[i] >= T0[i] + dT_ign)\n'
-        '        t_ign[i] = t_step;\n}\n\n')
+        # ---- RK4 stage combination for the batched integrator ------------
+        body.append(
+            '// One RK4 stage on the state rows 0..NSP-1 (row 0 = T), written\n'
+            '// without contraction so that it is the textbook sequence exactly:\n'
+            '//   1: acc = k,      ytmp = y + h/2 k\n'
+            '//   2: acc += 2 k,   ytmp = y + h/2 k\n'
+            '//   3: acc += 2 k,   ytmp = y + h k\n'
+            '//   4: y += h/6 (acc + k); a state whose T first reaches\n'
+            '//      T0 + dT_ign gets t_ign = t_step, the time at the start of\n'
+            '//      this step, as the reference records t before its t += h\n'
+            '//      (create_rate_subs.py:267-276).  NaN marks a state that has\n'
+            '//      not ignited yet, so any finite time (a negative t0 too) is\n'
+            '//      kept once set.\n'
+            '__global__ __launch_bounds__(crs::kBlock) void k_rk4_combine(\n'
+            '    double *__restrict__ y, double *__restrict__ ytmp,\n'
+            '    double *__restrict__ acc, const double *__restrict__ k,\n'
+            '    int64_t n, int64_t ld, int stage, double h, double t_step,\n'
+            '    const double *__restrict__ T0, double *__restrict__ t_ign,\n'
+            '    double dT_ign)\n{\n'
+            '#pragma clang fp contract(off)\n'
+            '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n'
+            '    if (i >= n) return;\n'
+            '    for (int r = 0; r < NSP; ++r) {\n'
+            '        const int64_t o = (int64_t)r * ld + i;\n'
+            '        const double kv = k[o];\n'
+            '        if (stage == 1) {\n'
+            '            acc[o] = kv;\n'
+            '            ytmp[o] = y[o] + 0.5 * h * kv;\n'
+            '        } else if (stage == 2) {\n'
+            '            acc[o] += 2.0 * kv;\n'
+            '            ytmp[o] = y[o] + 0.5 * h * kv;\n'
+            '        } else if (stage == 3) {\n'
+            '            acc[o] += 2.0 * kv;\n'
+            '            ytmp[o] = y[o] + h * kv;\n'
+            '        } else {\n'
+            '            y[o] += (h / 6.0) * (acc[o] + kv);\n'
+            '        }\n'
+            '    }\n'
+            '    if (stage == 4 && t_ign != nullptr && t_ign[i] != t_ign[i] &&\n'
+            '        y[i] >= T0[i] + dT_ign)\n'
+            '        t_ign[i] = t_step;\n}\n\n')
 
     # ---- fused mixed-precision per-state body -------------------------
     if opts.get('mixed', True):
@@ -2388,11 +2577,164 @@ def emit_hip_source(ir, opts=None):
         'constexpr int NREV = {nrev};\nconstexpr int NPD = {npd};\n\n'
         .format(name=ir.name, nsp=nsp, nr=nr, nrev=nrev, npd=npd))
     consts = K.definition() + '\n'
-    abi = _abi_text(ir, opts.get('mixed', True))
+    abi = _abi_text(ir, opts.get('mixed', True), dydt_split)
     return header + consts + ''.join(body) + '}  // namespace\n\n' + abi
 
 
-def _abi_text(ir, mixed=True):
+# crs_dydt and crs_rk4: the split form (wdot kernel, then the energy stage or
+# the energy stage with the RK4 combination) and the fused one (k_dydt_f64,
+# then k_rk4_combine)
+_DYDT_RK4_SPLIT = '''int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
+             int64_t ld, int mode, void *stream)
+{{
+    const char *fn = "crs_dydt";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (mode != CRS_MODE_FP64)
+        return crs::fail(mode == CRS_MODE_MIXED ? CRS_ERR_UNSUPPORTED
+                                                : CRS_ERR_ARG,
+                         "%s: mode %d not available (FP64 only)", fn, mode);
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, y, "y", n));
+    CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
+    if (n > 0 && dy == y)
+        return crs::fail(CRS_ERR_ARG, "%s: dy must not be y (the rates are "
+                         "staged in dy while y is read)", fn);
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    // the species rates of the state (T = row 0, Y = rows 1..) into dy,
+    // then the energy stage in place
+    for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
+        const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
+        k_wdot_f64<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
+            K, y + c0, P + c0, y + ld + c0, dy + c0, nullptr, c, nullptr, ld);
+        k_dydt_energy<<<crs::grid_for(c), crs::kBlock, 0, s>>>(
+            P + c0, y + c0, dy + c0, c, ld);
+    }}
+    return crs::launched(fn);
+}}
+
+int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
+            double t0, double h, int nsteps, const double *T0, double *t_ign,
+            double dT_ign, void *stream)
+{{
+    const char *fn = "crs_rk4";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (nsteps < 0)
+        return crs::fail(CRS_ERR_ARG, "%s: nsteps must be >= 0", fn);
+    if (!(h > 0.0))
+        return crs::fail(CRS_ERR_ARG, "%s: h must be > 0", fn);
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, y, "y", n));
+    CRS_TRY(crs::check_ptr(fn, work, "work", n));
+    if (n > crs::kMaxChunk)
+        return crs::fail(CRS_ERR_ARG, "%s: n = %lld exceeds %lld states per "
+                         "call", fn, (long long)n, (long long)crs::kMaxChunk);
+    if ((T0 == nullptr) != (t_ign == nullptr))
+        return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
+    if (n == 0 || nsteps == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    double *ytmp = work;
+    double *acc = work + (int64_t)NSP * ld;
+    double *wk = work + 2 * (int64_t)NSP * ld;
+    const dim3 grid = crs::grid_for(n);
+    const dim3 wgrid = crs::grid_for(n, kWdotBlock);
+    double t = t0;
+    for (int step = 0; step < nsteps; ++step) {{
+        for (int stage = 1; stage <= 4; ++stage) {{
+            const double *ys = stage == 1 ? y : ytmp;
+            k_wdot_f64<<<wgrid, kWdotBlock, 0, s>>>(K, ys, P, ys + ld, wk,
+                                                     nullptr, n, nullptr, ld);
+            if (stage == 1)
+                k_rk4_stage<1><<<grid, crs::kBlock, 0, s>>>(
+                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+            else if (stage == 2)
+                k_rk4_stage<2><<<grid, crs::kBlock, 0, s>>>(
+                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+            else if (stage == 3)
+                k_rk4_stage<3><<<grid, crs::kBlock, 0, s>>>(
+                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+            else
+                k_rk4_stage<4><<<grid, crs::kBlock, 0, s>>>(
+                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+        }}
+        t += h;
+    }}
+    return crs::launched(fn);
+}}
+
+'''
+
+_DYDT_RK4_FUSED = '''int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
+             int64_t ld, int mode, void *stream)
+{{
+    const char *fn = "crs_dydt";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (mode != CRS_MODE_FP64)
+        return crs::fail(mode == CRS_MODE_MIXED ? CRS_ERR_UNSUPPORTED
+                                                : CRS_ERR_ARG,
+                         "%s: mode %d not available (FP64 only)", fn, mode);
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, y, "y", n));
+    CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
+    if (n == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
+        const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
+        k_dydt_f64<<<crs::grid_for(c), crs::kBlock, 0, s>>>(
+            K, P + c0, y + c0, dy + c0, c, ld);
+    }}
+    return crs::launched(fn);
+}}
+
+int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
+            double t0, double h, int nsteps, const double *T0, double *t_ign,
+            double dT_ign, void *stream)
+{{
+    const char *fn = "crs_rk4";
+    CRS_TRY(crs::check_sizes(fn, n, ld));
+    if (nsteps < 0)
+        return crs::fail(CRS_ERR_ARG, "%s: nsteps must be >= 0", fn);
+    if (!(h > 0.0))
+        return crs::fail(CRS_ERR_ARG, "%s: h must be > 0", fn);
+    CRS_TRY(crs::check_ptr(fn, P, "P", n));
+    CRS_TRY(crs::check_ptr(fn, y, "y", n));
+    CRS_TRY(crs::check_ptr(fn, work, "work", n));
+    if (n > crs::kMaxChunk)
+        return crs::fail(CRS_ERR_ARG, "%s: n = %lld exceeds %lld states per "
+                         "call", fn, (long long)n, (long long)crs::kMaxChunk);
+    if ((T0 == nullptr) != (t_ign == nullptr))
+        return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
+    if (n == 0 || nsteps == 0) return CRS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *K = nullptr;
+    CRS_TRY(table(fn, &K));
+    double *ytmp = work;
+    double *acc = work + (int64_t)NSP * ld;
+    double *k = work + 2 * (int64_t)NSP * ld;
+    const dim3 grid = crs::grid_for(n);
+    double t = t0;
+    for (int step = 0; step < nsteps; ++step) {{
+        for (int stage = 1; stage <= 4; ++stage) {{
+            k_dydt_f64<<<grid, crs::kBlock, 0, s>>>(K, P, stage == 1 ? y : ytmp,
+                                                    k, n, ld);
+            k_rk4_combine<<<grid, crs::kBlock, 0, s>>>(
+                y, ytmp, acc, k, n, ld, stage, h, t, T0, t_ign, dT_ign);
+        }}
+        t += h;
+    }}
+    return crs::launched(fn);
+}}
+
+'''
+
+
+def _abi_text(ir, mixed=True, dydt_split=True):
     npd = ir.npd
     modes = '(1 << CRS_MODE_FP64)' + (' | (1 << CRS_MODE_MIXED)' if mixed else '')
     if mixed:
@@ -2565,70 +2907,7 @@ int crs_eval_wdot(const double *T, const double *P, const double *Y,
     return crs::launched(fn);
 }}
 
-int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
-             int64_t ld, int mode, void *stream)
-{{
-    const char *fn = "crs_dydt";
-    CRS_TRY(crs::check_sizes(fn, n, ld));
-    if (mode != CRS_MODE_FP64)
-        return crs::fail(mode == CRS_MODE_MIXED ? CRS_ERR_UNSUPPORTED
-                                                : CRS_ERR_ARG,
-                         "%s: mode %d not available (FP64 only)", fn, mode);
-    CRS_TRY(crs::check_ptr(fn, P, "P", n));
-    CRS_TRY(crs::check_ptr(fn, y, "y", n));
-    CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
-    if (n == 0) return CRS_OK;
-    hipStream_t s = (hipStream_t)stream;
-    const double *K = nullptr;
-    CRS_TRY(table(fn, &K));
-    for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
-        const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
-        k_dydt_f64<<<crs::grid_for(c), crs::kBlock, 0, s>>>(
-            K, P + c0, y + c0, dy + c0, c, ld);
-    }}
-    return crs::launched(fn);
-}}
-
-int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
-            double t0, double h, int nsteps, const double *T0, double *t_ign,
-            double dT_ign, void *stream)
-{{
-    const char *fn = "crs_rk4";
-    CRS_TRY(crs::check_sizes(fn, n, ld));
-    if (nsteps < 0)
-        return crs::fail(CRS_ERR_ARG, "%s: nsteps must be >= 0", fn);
-    if (!(h > 0.0))
-        return crs::fail(CRS_ERR_ARG, "%s: h must be > 0", fn);
-    CRS_TRY(crs::check_ptr(fn, P, "P", n));
-    CRS_TRY(crs::check_ptr(fn, y, "y", n));
-    CRS_TRY(crs::check_ptr(fn, work, "work", n));
-    if (n > crs::kMaxChunk)
-        return crs::fail(CRS_ERR_ARG, "%s: n = %lld exceeds %lld states per "
-                         "call", fn, (long long)n, (long long)crs::kMaxChunk);
-    if ((T0 == nullptr) != (t_ign == nullptr))
-        return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
-    if (n == 0 || nsteps == 0) return CRS_OK;
-    hipStream_t s = (hipStream_t)stream;
-    const double *K = nullptr;
-    CRS_TRY(table(fn, &K));
-    double *ytmp = work;
-    double *acc = work + (int64_t)NSP * ld;
-    double *k = work + 2 * (int64_t)NSP * ld;
-    const dim3 grid = crs::grid_for(n);
-    double t = t0;
-    for (int step = 0; step < nsteps; ++step) {{
-        for (int stage = 1; stage <= 4; ++stage) {{
-            k_dydt_f64<<<grid, crs::kBlock, 0, s>>>(K, P, stage == 1 ? y : ytmp,
-                                                    k, n, ld);
-            k_rk4_combine<<<grid, crs::kBlock, 0, s>>>(
-                y, ytmp, acc, k, n, ld, stage, h, t, T0, t_ign, dT_ign);
-        }}
-        t += h;
-    }}
-    return crs::launched(fn);
-}}
-
-// Shared by crs_eval_wdot_subset (host count m, m_dev NULL) and
+{dydt_rk4}// Shared by crs_eval_wdot_subset (host count m, m_dev NULL) and
 // crs_eval_wdot_subset_dev (at most m states, the count read on the device).
 static int wdot_subset(const char *fn, const double *T, const double *P,
                        const double *Y, double *wdot, const int32_t *idx,
@@ -2679,7 +2958,9 @@ int crs_eval_wdot_subset_dev(const double *T, const double *P,
 
 }}  // extern "C"
 '''.format(name=ir.name, pm_launch=pm_launch, modes=modes,
-           launch_full_c=launch_full, launch_sub=launch_sub, mixed_ok=mixed_ok)
+           launch_full_c=launch_full, launch_sub=launch_sub, mixed_ok=mixed_ok,
+           dydt_rk4=(_DYDT_RK4_SPLIT if dydt_split else
+                     _DYDT_RK4_FUSED).replace('{{', '{').replace('}}', '}'))
 
 
 def write_hip(out_dir, specs, reacs, name='mech', spec_order=None, opts=None,

@@ -25,7 +25,11 @@ def _ir(name):
 
 
 def _bodies(name, opts=None):
-    text = hip_emitter.emit_hip_source(_ir(name), opts)
+    """Source and per-state bodies, with the fused dydt kernel emitted
+    (option dydt='fused') so that its body is checked beside the wdot
+    ones."""
+    text = hip_emitter.emit_hip_source(_ir(name),
+                                       dict(opts or {}, dydt='fused'))
     out = {}
     for fn in _BODIES:
         i = text.index('void ' + fn)
@@ -248,3 +252,45 @@ def test_fused_kernels_use_the_transforms():
     body = plain[plain.index('void wdot_f64_state'):
                  plain.index('void k_wdot_f64')]
     assert 'crs::go<' not in body and 'Y[0 * ld + i]' in body
+
+
+def test_split_dydt_is_the_default():
+    """By default crs_dydt runs the wdot kernel into dy and then the energy
+    stage in place, crs_rk4 the wdot kernel and the energy + RK4-stage
+    kernel; the fused dydt kernel is not emitted."""
+    text = hip_emitter.emit_hip_source(_ir('mech53'))
+    assert 'void k_dydt_f64(' not in text
+    assert 'void k_rk4_combine(' not in text
+    assert 'void k_dydt_energy(' in text and 'void k_rk4_stage(' in text
+    dydt = text[text.index('int crs_dydt('):text.index('int crs_rk4(')]
+    w = dydt.index('k_wdot_f64<<<')
+    e = dydt.index('k_dydt_energy<<<')
+    assert w < e
+    assert 'dy == y' in dydt
+    rk4 = text[text.index('int crs_rk4('):]
+    rk4 = rk4[:rk4.index('\n}\n')]
+    assert rk4.index('k_wdot_f64<<<') < rk4.index('k_rk4_stage<1><<<')
+    # the energy stage is not contracted, so the RK4 kernel's stage
+    # derivatives are the bits crs_dydt returns
+    body = text[text.index('void dydt_energy_state('):]
+    body = body[:body.index('\n}\n')]
+    assert '#pragma clang fp contract(off)' in body
+    # rates in chunks from the last species down; within a chunk every
+    # rate is loaded before the first output, and a chunk writes rows k+1
+    # whose rates (row k+1) were loaded by this chunk or an earlier one
+    # (W may be dy itself)
+    loaded = set()
+    for line in body.split('\n'):
+        m = re.search(r'const double w(\d+) = crs::ld_row\(W \+ (\d+) \* ld', line)
+        if m:
+            assert m.group(1) == m.group(2)
+            loaded.add(int(m.group(1)))
+        m = re.search(r'out\.put\((\d+),', line)
+        if m:
+            assert int(m.group(1)) in loaded, line
+    assert loaded == set(range(53))
+    fused = hip_emitter.emit_hip_source(_ir('mech53'), {'dydt': 'fused'})
+    assert 'void k_dydt_f64(' in fused and 'void k_rk4_combine(' in fused
+    assert 'void k_dydt_energy(' not in fused
+    with pytest.raises(ValueError):
+        hip_emitter.emit_hip_source(_ir('mech13'), {'dydt': 'other'})
