@@ -266,7 +266,7 @@ PROBE_LIB = os.path.join(workloads.BUILD_ROOT, 'libcrs_probe.so')
 
 
 def build_probe(verbose=False):
-    """Compile the FP64 FMA-ceiling probe (tune.ceiling)."""
+    """Compile the ceiling probes (tune.ceiling, tune.bandwidth)."""
     os.makedirs(workloads.BUILD_ROOT, exist_ok=True)
     cmd = [hipcc()] + HIPCC_FLAGS + ['-o', PROBE_LIB + '.tmp', PROBE_SRC]
     res = subprocess.run(cmd, capture_output=True, text=True)

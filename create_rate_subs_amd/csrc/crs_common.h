@@ -13,6 +13,7 @@
 #include <mutex>
 
 #include "crs_hip.h"
+#include "crs_overlap.h"
 
 namespace crs {
 
@@ -63,6 +64,15 @@ inline int check_ptr(const char *fn, const void *p, const char *what, int64_t n)
 {
     if (n > 0 && p == nullptr)
         return fail(CRS_ERR_ARG, "%s: %s is NULL", fn, what);
+    return CRS_OK;
+}
+
+inline int check_disjoint(const char *fn, const void *a, int64_t ra,
+                          const char *wa, const void *b, int64_t rb,
+                          const char *wb, int64_t n, int64_t ld)
+{
+    if (rows_overlap(a, ra, b, rb, n, ld))
+        return fail(CRS_ERR_ARG, "%s: %s and %s overlap", fn, wa, wb);
     return CRS_OK;
 }
 

@@ -2596,9 +2596,9 @@ _DYDT_RK4_SPLIT = '''int crs_dydt(const double *P, const double *y, double *dy, 
     CRS_TRY(crs::check_ptr(fn, P, "P", n));
     CRS_TRY(crs::check_ptr(fn, y, "y", n));
     CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
-    if (n > 0 && dy == y)
-        return crs::fail(CRS_ERR_ARG, "%s: dy must not be y (the rates are "
-                         "staged in dy while y is read)", fn);
+    // dy is written while y and P are still read
+    CRS_TRY(crs::check_disjoint(fn, y, NSP, "y", dy, NSP, "dy", n, ld));
+    CRS_TRY(crs::check_disjoint(fn, P, 1, "P", dy, NSP, "dy", n, ld));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
@@ -2633,6 +2633,21 @@ int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
                          "call", fn, (long long)n, (long long)crs::kMaxChunk);
     if ((T0 == nullptr) != (t_ign == nullptr))
         return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
+    // y, work and t_ign are written while P, T0 and each other are read:
+    // no two of them may share an entry
+    CRS_TRY(crs::check_disjoint(fn, y, NSP, "y", work, 3 * NSP, "work", n, ld));
+    CRS_TRY(crs::check_disjoint(fn, P, 1, "P", y, NSP, "y", n, ld));
+    CRS_TRY(crs::check_disjoint(fn, P, 1, "P", work, 3 * NSP, "work", n, ld));
+    if (t_ign) {{
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", y, NSP, "y", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", work, 3 * NSP,
+                                    "work", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", P, 1, "P", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", T0, 1, "T0", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, T0, 1, "T0", y, NSP, "y", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, T0, 1, "T0", work, 3 * NSP, "work",
+                                    n, ld));
+    }}
     if (n == 0 || nsteps == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
@@ -2680,6 +2695,9 @@ _DYDT_RK4_FUSED = '''int crs_dydt(const double *P, const double *y, double *dy, 
     CRS_TRY(crs::check_ptr(fn, P, "P", n));
     CRS_TRY(crs::check_ptr(fn, y, "y", n));
     CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
+    // dy is written while y and P are still read
+    CRS_TRY(crs::check_disjoint(fn, y, NSP, "y", dy, NSP, "dy", n, ld));
+    CRS_TRY(crs::check_disjoint(fn, P, 1, "P", dy, NSP, "dy", n, ld));
     if (n == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
@@ -2710,6 +2728,21 @@ int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
                          "call", fn, (long long)n, (long long)crs::kMaxChunk);
     if ((T0 == nullptr) != (t_ign == nullptr))
         return crs::fail(CRS_ERR_ARG, "%s: T0 and t_ign go together", fn);
+    // y, work and t_ign are written while P, T0 and each other are read:
+    // no two of them may share an entry
+    CRS_TRY(crs::check_disjoint(fn, y, NSP, "y", work, 3 * NSP, "work", n, ld));
+    CRS_TRY(crs::check_disjoint(fn, P, 1, "P", y, NSP, "y", n, ld));
+    CRS_TRY(crs::check_disjoint(fn, P, 1, "P", work, 3 * NSP, "work", n, ld));
+    if (t_ign) {{
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", y, NSP, "y", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", work, 3 * NSP,
+                                    "work", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", P, 1, "P", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, t_ign, 1, "t_ign", T0, 1, "T0", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, T0, 1, "T0", y, NSP, "y", n, ld));
+        CRS_TRY(crs::check_disjoint(fn, T0, 1, "T0", work, 3 * NSP, "work",
+                                    n, ld));
+    }}
     if (n == 0 || nsteps == 0) return CRS_OK;
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;

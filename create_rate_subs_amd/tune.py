@@ -15,6 +15,8 @@ conditioned error of create_rate_subs_amd.metrics:
     python -m create_rate_subs_amd.tune sweep mech53
     # on the GPU: plain FP64 FMA chains, the VALU issue ceiling
     python -m create_rate_subs_amd.tune ceiling
+    # on the GPU: plain copy and the energy stage's row pattern, HBM rate
+    python -m create_rate_subs_amd.tune bandwidth
 
 Variants live under _build/variants/<mechanism>/<name>/ with the options
 they were built from (opts.json).  One JSON line per (variant, mode).
@@ -239,6 +241,55 @@ def ceiling(ks=(1, 2, 4, 8), waves=(1, 2, 4), blocks=256, n=8192,
                   flush=True)
 
 
+def bandwidth(n=1 << 20, rows=53, launches=20):
+    """HBM streaming rates (csrc/fma_probe.hip): a plain copy of rows x n
+    doubles, and the dydt energy stage's row pattern (rows read from Y and
+    from W, rows written to dy, per state) with 1, 2 and 4 states per lane.
+    One JSON line each; TB/s counts every byte once."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(build.PROBE_LIB)
+    lib.crs_probe_copy.restype = ctypes.c_int
+    lib.crs_probe_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int64, ctypes.c_void_p]
+    lib.crs_probe_rows.restype = ctypes.c_int
+    lib.crs_probe_rows.argtypes = [ctypes.c_void_p] * 3 + [
+        ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    stream = torch.cuda.current_stream()
+    Y, W, dy = (torch.zeros((rows, n), dtype=torch.float64, device='cuda')
+                for _ in range(3))
+    nbytes = Y.numel() * 8
+
+    def timed(launch):
+        launch()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(launches):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / launches
+
+    def copy():
+        if lib.crs_probe_copy(Y.data_ptr(), dy.data_ptr(), nbytes,
+                              stream.cuda_stream):
+            raise RuntimeError('crs_probe_copy failed')
+    ms = timed(copy)
+    print(json.dumps(dict(kernel='copy', bytes=2 * nbytes, ms=ms,
+                          TBs=2 * nbytes / (ms * 1e-3) / 1e12)), flush=True)
+    for per_lane in (1, 2, 4):
+        def pattern():
+            if lib.crs_probe_rows(Y.data_ptr(), W.data_ptr(), dy.data_ptr(),
+                                  n, n, per_lane, stream.cuda_stream):
+                raise RuntimeError('crs_probe_rows failed')
+        ms = timed(pattern)
+        print(json.dumps(dict(kernel='rows', states_per_lane=per_lane,
+                              bytes=3 * nbytes, ms=ms,
+                              TBs=3 * nbytes / (ms * 1e-3) / 1e12)),
+              flush=True)
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     if len(argv) >= 4 and argv[0] == 'build':
@@ -249,6 +300,8 @@ def main(argv=None):
         sweep(argv[1])
     elif argv and argv[0] == 'ceiling':
         ceiling()
+    elif argv and argv[0] == 'bandwidth':
+        bandwidth()
     else:
         print(__doc__)
         return 2

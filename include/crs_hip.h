@@ -156,7 +156,11 @@ int crs_eval_wdot_subset_dev(const double *T, const double *P,
  * dY_k/dt = wdot_k W_k / rho.  Any row beyond NSP-1 (an integrator's NN-th
  * slot) is neither read nor written, as in the reference.  The time
  * argument of the reference is unused there and omitted here.  FP64 only
- * (CRS_MODE_MIXED returns CRS_ERR_UNSUPPORTED).
+ * (CRS_MODE_MIXED returns CRS_ERR_UNSUPPORTED).  Two kernels per call: the
+ * fused wdot kernel writes the species rates into dy, then an energy
+ * kernel overwrites dy in place with the derivatives.  dy therefore must
+ * share no entry with y or P (they may share an allocation, e.g. side by
+ * side within rows); an overlap returns CRS_ERR_ARG before any launch.
  */
 int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
              int64_t ld, int mode, void *stream);
@@ -175,8 +179,10 @@ int crs_dydt(const double *P, const double *y, double *dy, int64_t n,
  * the end of a step gets t_ign = the time at the START of that step (the
  * reference sets t_ign = t at :271 before its `t += h` at :276); step k
  * starts at t0 plus k additions of h, as the reference accumulates t.
- * n is at most 2^29 states per call.
- * Enqueues 8 kernels per step on the stream (capturable in a graph; no
+ * n is at most 2^29 states per call.  y, work, P, T0 and t_ign must share
+ * no entry (CRS_ERR_ARG otherwise, before any launch).
+ * Enqueues 8 kernels per step on the stream, per RK4 stage the fused wdot
+ * kernel and one energy-and-stage kernel (capturable in a graph; no
  * memset, no allocation).
  */
 int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,

@@ -13,7 +13,9 @@
 //     own text back) and per library (a surrogate-library failure leaves
 //     the mechanism library's message alone);
 //   - no out-of-bounds access, use after free or undefined behaviour in
-//     the shim code (the sanitizers abort the run on the first).
+//     the shim code (the sanitizers abort the run on the first);
+//   - the overlap test behind the aliasing checks of crs_dydt / crs_rk4
+//     (crs::rows_overlap) agrees with the byte ranges on every small case.
 // Prints "ok" and exits 0 when every check passes.
 #include <atomic>
 #include <cmath>
@@ -26,6 +28,7 @@
 
 #include "crs_hip.h"
 #include "crs_mlp.h"
+#include "crs_overlap.h"
 
 namespace {
 
@@ -65,7 +68,11 @@ void mechanism_library(const crs_mech_info &info)
     // before touching them
     alignas(64) static double d[64];
     alignas(64) static int32_t ix[64];
+    // a [4 NSP][4] state block and three more of its size, for the overlap
+    // checks (addresses only)
+    static std::vector<double> rows(16 * (size_t)info.nsp + 64);
     double *p = d;
+    double *q = rows.data();
     const int32_t *ip = ix;
     const int64_t big = ((int64_t)1 << 29) + 1;
 
@@ -144,6 +151,17 @@ void mechanism_library(const crs_mech_info &info)
          "y is NULL");
     MECH(crs_dydt(nullptr, nullptr, nullptr, 0, 0, CRS_MODE_FP64, nullptr),
          CRS_OK, nullptr);
+    // dy written while y and P are read: any shared byte is refused (rows
+    // of 4 entries at stride 4: d[0..4 NSP) against d + 1 and d + 4 NSP - 1)
+    const int64_t nsp = info.nsp;
+    MECH(crs_dydt(p + 40, p, p, 4, 4, CRS_MODE_FP64, nullptr), CRS_ERR_ARG,
+         "crs_dydt: y and dy overlap");
+    MECH(crs_dydt(p + 40, q, q + 1, 4, 4, CRS_MODE_FP64, nullptr),
+         CRS_ERR_ARG, "y and dy overlap");
+    MECH(crs_dydt(p + 40, q, q + 4 * nsp - 1, 4, 4, CRS_MODE_FP64, nullptr),
+         CRS_ERR_ARG, "y and dy overlap");
+    MECH(crs_dydt(q + 4, q + 4 * nsp + 8, q, 4, 4, CRS_MODE_FP64, nullptr),
+         CRS_ERR_ARG, "P and dy overlap");
 
     MECH(crs_rk4(p, p, p, 4, 4, 0.0, 1e-9, -1, nullptr, nullptr, 400.0,
                  nullptr), CRS_ERR_ARG, "crs_rk4: nsteps must be >= 0");
@@ -157,10 +175,61 @@ void mechanism_library(const crs_mech_info &info)
          CRS_ERR_ARG, "T0 and t_ign go together");
     MECH(crs_rk4(p, p, nullptr, 4, 4, 0.0, 1e-9, 1, nullptr, nullptr, 400.0,
                  nullptr), CRS_ERR_ARG, "work is NULL");
-    MECH(crs_rk4(p, p, p, 4, 4, 0.0, 1e-9, 0, nullptr, nullptr, 400.0,
-                 nullptr), CRS_OK, nullptr);
+    MECH(crs_rk4(p, q, q + 4 * nsp, 4, 4, 0.0, 1e-9, 0, nullptr, nullptr,
+                 400.0, nullptr), CRS_OK, nullptr);
+    MECH(crs_rk4(p, q, q + 4 * nsp - 2, 4, 4, 0.0, 1e-9, 1, nullptr,
+                 nullptr, 400.0, nullptr), CRS_ERR_ARG,
+         "crs_rk4: y and work overlap");
+    MECH(crs_rk4(q + 4 * nsp, q, q + 4 * nsp, 4, 4, 0.0, 1e-9, 1, nullptr,
+                 nullptr, 400.0, nullptr), CRS_ERR_ARG, "P and work overlap");
+    MECH(crs_rk4(p, q, q + 4 * nsp, 4, 4, 0.0, 1e-9, 1, p + 8, p + 10, 400.0,
+                 nullptr), CRS_ERR_ARG, "t_ign and T0 overlap");
+    MECH(crs_rk4(p, q, q + 4 * nsp, 4, 4, 0.0, 1e-9, 1, p + 8, q + 2, 400.0,
+                 nullptr), CRS_ERR_ARG, "t_ign and y overlap");
+    MECH(crs_rk4(p, q, q + 4 * nsp, 4, 4, 0.0, 1e-9, 1, q + 1, p + 20, 400.0,
+                 nullptr), CRS_ERR_ARG, "T0 and y overlap");
     MECH(crs_rk4(nullptr, nullptr, nullptr, 0, 0, 0.0, 1e-9, 3, nullptr,
                  nullptr, 400.0, nullptr), CRS_OK, nullptr);
+}
+
+// crs::rows_overlap against the byte ranges themselves: every block shape
+// with n <= 3, n <= ld <= 5, up to 3 rows each, at every byte offset of b
+// from a within +-4 rows (aligned or not).
+void overlap_brute_force()
+{
+    alignas(64) static char buf[1024];
+    const char *a = buf + 512;
+    int checked = 0;
+    for (int64_t n = 1; n <= 3; ++n)
+        for (int64_t ld = n; ld <= 5; ++ld)
+            for (int64_t ra = 1; ra <= 3; ++ra)
+                for (int64_t rb = 1; rb <= 3; ++rb)
+                    for (int64_t d = -32 * ld; d <= 32 * ld; ++d) {
+                        bool truth = false;
+                        for (int64_t r1 = 0; r1 < ra; ++r1)
+                            for (int64_t i1 = 0; i1 < n; ++i1)
+                                for (int64_t r2 = 0; r2 < rb; ++r2)
+                                    for (int64_t i2 = 0; i2 < n; ++i2) {
+                                        const int64_t x = 8 * (r1 * ld + i1);
+                                        const int64_t y = d + 8 * (r2 * ld + i2);
+                                        if (x < y + 8 && y < x + 8) truth = true;
+                                    }
+                        const bool got = crs::rows_overlap(a, ra, a + d, rb,
+                                                           n, ld);
+                        if (got != truth) {
+                            std::fprintf(stderr, "rows_overlap(n=%lld ld=%lld "
+                                         "ra=%lld rb=%lld d=%lld) = %d, "
+                                         "bytes say %d\n", (long long)n,
+                                         (long long)ld, (long long)ra,
+                                         (long long)rb, (long long)d, got,
+                                         truth);
+                            ++failures;
+                            return;
+                        }
+                        ++checked;
+                    }
+    std::printf("rows_overlap: %d cases agree with the byte ranges\n",
+                checked);
 }
 
 void surrogate_library()
@@ -394,6 +463,7 @@ int main()
     std::printf("library %s: nsp %d nr %d nrev %d npd %d modes %d\n",
                 info.name, info.nsp, info.nr, info.nrev, info.npd,
                 info.modes);
+    overlap_brute_force();
     mechanism_library(info);
     surrogate_library();
     per_thread_messages();

@@ -101,7 +101,8 @@ def build_abi_san(name):
     exe = os.path.join(d, 'abi_san')
     clang = os.path.join(ROCM, 'llvm', 'bin', 'clang++')
     _run([clang, '-std=c++17', '-O1'] + SAN +
-         ['-I', os.path.join(ROOT, 'include'), '-o', exe,
+         ['-I', os.path.join(ROOT, 'include'), '-I',
+          os.path.join(ROOT, 'create_rate_subs_amd', 'csrc'), '-o', exe,
           os.path.join(HERE, 'abi_san.cpp'), '-L', d,
           '-lcrs_{}'.format(name), '-lcrs_mlp', '-pthread',
           '-Wl,-rpath,' + d, '-Wl,-rpath,' + os.path.join(ROCM, 'lib')],

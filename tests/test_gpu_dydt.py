@@ -107,3 +107,33 @@ def test_dydt_rejects_mixed_mode_and_bad_sizes():
     out = lib.dydt(e, torch.empty((lib.nsp, 0), dtype=torch.float64,
                                   device='cuda'))
     assert out.shape == (lib.nsp, 0)
+
+
+def test_dydt_interleaved_buffers_and_overlap_refused():
+    """y and dy may share an allocation as long as no entry is shared
+    (here side by side within each row, row stride 2n); any shared entry
+    is refused before a launch, and so is work overlapping y in crs_rk4."""
+    import torch
+    from create_rate_subs_amd.runtime import CrsError
+    name = 'mech53'
+    n = 1000
+    T, P, Y, y = _inputs(name, n, 45)
+    lib = hip_lib(name)
+    nsp = lib.nsp
+    compact = to_host(lib.dydt(*to_dev(P, y)))
+    Pd, yd = to_dev(P, y)
+    buf = torch.full((nsp, 2 * n), 7.0, dtype=torch.float64, device='cuda')
+    buf[:, :n] = yd
+    lib.dydt(Pd, buf[:, :n], out=buf[:, n:])
+    got = to_host(buf)
+    np.testing.assert_array_equal(got[:, n:], compact)
+    np.testing.assert_array_equal(got[:, :n], y)
+    for out in (buf[:, :n], buf[:, 1:n + 1], buf[:, n - 1:2 * n - 1]):
+        with pytest.raises(CrsError, match='y and dy overlap'):
+            lib.dydt(Pd, buf[:, :n], out=out)
+    np.testing.assert_array_equal(to_host(buf), got)
+    blk = torch.zeros((4 * nsp, n), dtype=torch.float64, device='cuda')
+    blk[:nsp] = yd
+    with pytest.raises(CrsError, match='y and work overlap'):
+        lib.rk4(Pd, blk[:nsp], 0.0, 1e-12, 1, work=blk[nsp - 1:4 * nsp - 1])
+    np.testing.assert_array_equal(to_host(blk[:nsp]), y)

@@ -20,8 +20,9 @@ enough to stay finite:
 
 * RK4 on the GPU is bit-identical to the textbook RK4 combination done on
   the host with the stage derivatives the GPU's crs_dydt returns for the
-  host's own stage states (the combination kernel is not contracted), for
-  1, 2 and 3 steps;
+  host's own stage states (the energy-and-stage kernels are not
+  contracted, so their derivatives are crs_dydt's bits), for 1, 2 and 3
+  steps;
 * the harness' captured HIP graph, replayed several times from the same
   initial state with eager work and synchronisations in between, gives the
   eager result every time;

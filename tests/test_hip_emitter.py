@@ -266,7 +266,7 @@ def test_split_dydt_is_the_default():
     w = dydt.index('k_wdot_f64<<<')
     e = dydt.index('k_dydt_energy<<<')
     assert w < e
-    assert 'dy == y' in dydt
+    assert 'check_disjoint(fn, y, NSP, "y", dy, NSP, "dy"' in dydt
     rk4 = text[text.index('int crs_rk4('):]
     rk4 = rk4[:rk4.index('\n}\n')]
     assert rk4.index('k_wdot_f64<<<') < rk4.index('k_rk4_stage<1><<<')
