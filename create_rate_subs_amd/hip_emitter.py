@@ -722,9 +722,6 @@ DEFAULT_OPTS = dict(consts='literal', sched_barrier=False, early_store=True,
                     fused='fast', kc='auto', exp='crs', order='retire',
                     nasa='auto', waves_per_simd='auto')
 
-# per-lane G rows per wave at two waves per SIMD (two 4-wave blocks per CU)
-_ROWS_2W = (80 * 1024 - 512) // 2048
-
 _LDS_PER_CU = 160 * 1024
 
 
