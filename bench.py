@@ -32,7 +32,8 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
-    # enough warm-up launches (tens of ms) for the clocks to settle
+    # untimed launches before each timed window; at least WARM_S seconds
+    # of them run whatever the count
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--workload', default='w1_53')
     ap.add_argument('--states', type=int, default=0,
@@ -107,11 +108,32 @@ def measured_traffic(kernel, workload, n):
     return None, None
 
 
-def timed(torch, fn, steps, warmup):
-    """Run fn warmup times, then time steps launches with events and wall."""
+# Least wall time of untimed warm-up before a timed window.  After the part
+# has idled (at the start of the run, after the CPU baseline) its clocks
+# take tens of milliseconds of work to ramp: the first ~20 launches of the
+# headline kernel average 0.71 ms against 0.62 ms once warm, so a count of
+# warm-up launches alone does not reach the steady state.
+WARM_S = 0.5
+
+
+def timed(torch, fn, steps, warmup, info=None):
+    """Run fn warmup times, and on (untimed, 8 launches at a time) until
+    WARM_S seconds have passed since the first; then time exactly steps
+    launches with events and wall.  Returns (wall ms per launch, mean event
+    ms per launch); ``info``, a dict, receives the warm-up launch count."""
+    t0 = time.perf_counter()
+    done = 0
     for _ in range(warmup):
         fn()
+        done += 1
     torch.cuda.synchronize()
+    while time.perf_counter() - t0 < WARM_S:
+        for _ in range(8):
+            fn()
+        done += 8
+        torch.cuda.synchronize()
+    if info is not None:
+        info['warmup_launches'] = done
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     t0 = time.perf_counter()
@@ -435,9 +457,10 @@ def main():
                               surrogate=sur is not None)))
         return
 
+    warm = {}
     ms_per_step, kern_ms = timed(
         torch, lambda: lib.eval_wdot(Td, Pd, Yd, 'fp64', out=Wd),
-        args.steps, args.warmup)
+        args.steps, args.warmup, info=warm)
     value = n / (ms_per_step * 1e-3)
     log('fp64 {}: {:.3e} states/s'.format(wl.name, value))
 
@@ -461,6 +484,7 @@ def main():
                        'surrogate @ stated rel err',
                 value=value, unit='states/s', n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=ms_per_step,
+                warmup_launches=warm['warmup_launches'], warmup_min_s=WARM_S,
                 higher_is_better=True, scaling='weak', vs_baseline=None,
                 dtype='f64', data='synthetic',
                 config=dict(workload='{}: {} sp / {} rxn, {} FP64 states, '
