@@ -2586,10 +2586,7 @@ _DYDT_RK4_SPLIT = '''int crs_dydt(const double *P, const double *y, double *dy, 
 {{
     const char *fn = "crs_dydt";
     CRS_TRY(crs::check_sizes(fn, n, ld));
-    if (mode != CRS_MODE_FP64)
-        return crs::fail(mode == CRS_MODE_MIXED ? CRS_ERR_UNSUPPORTED
-                                                : CRS_ERR_ARG,
-                         "%s: mode %d not available (FP64 only)", fn, mode);
+    CRS_TRY(check_mode(fn, mode));
     CRS_TRY(crs::check_ptr(fn, P, "P", n));
     CRS_TRY(crs::check_ptr(fn, y, "y", n));
     CRS_TRY(crs::check_ptr(fn, dy, "dy", n));
@@ -2601,10 +2598,10 @@ _DYDT_RK4_SPLIT = '''int crs_dydt(const double *P, const double *y, double *dy, 
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
     // the species rates of the state (T = row 0, Y = rows 1..) into dy,
-    // then the energy stage in place
+    // from the mode's fused kernel, then the FP64 energy stage in place
     for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
         const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
-        k_wdot_f64<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
+@MIXED_RATES@        k_wdot_f64<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
             K, y + c0, P + c0, y + ld + c0, dy + c0, nullptr, c, nullptr, ld);
         k_dydt_energy<<<crs::grid_for(c), crs::kBlock, 0, s>>>(
             P + c0, y + c0, dy + c0, c, ld);
@@ -2779,8 +2776,14 @@ def _abi_text(ir, mixed=True, dydt_split=True):
     else
 '''
         mixed_ok = 'mode == CRS_MODE_MIXED'
+        mixed_rates = '''        if (mode == CRS_MODE_MIXED)
+            k_wdot_mixed<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
+                K, y + c0, P + c0, y + ld + c0, dy + c0, nullptr, c, nullptr,
+                ld);
+        else
+'''
     else:
-        launch_full = launch_sub = ''
+        launch_full = launch_sub = mixed_rates = ''
         mixed_ok = 'false'
     pm_launch = ('    k_get_rxn_pres_mod<<<crs::grid_for(n), crs::kBlock, 0, s>>>'
                  '(K, T, P, C, pres_mod, n, ld);\n' if npd else '')
@@ -2989,8 +2992,9 @@ int crs_eval_wdot_subset_dev(const double *T, const double *P,
 }}  // extern "C"
 '''.format(name=ir.name, pm_launch=pm_launch, modes=modes,
            launch_full_c=launch_full, launch_sub=launch_sub, mixed_ok=mixed_ok,
-           dydt_rk4=(_DYDT_RK4_SPLIT if dydt_split else
-                     _DYDT_RK4_FUSED).replace('{{', '{').replace('}}', '}'))
+           dydt_rk4=(_DYDT_RK4_SPLIT.replace('@MIXED_RATES@', mixed_rates)
+                     if dydt_split else _DYDT_RK4_FUSED)
+           .replace('{{', '{').replace('}}', '}'))
 
 
 def write_hip(out_dir, specs, reacs, name='mech', spec_order=None, opts=None,

@@ -155,10 +155,12 @@ int crs_eval_wdot_subset_dev(const double *T, const double *P,
  * dy row 0 = dT/dt = -sum_k wdot_k h_k W_k / (rho cp), rows 1..NSP-1 =
  * dY_k/dt = wdot_k W_k / rho.  Any row beyond NSP-1 (an integrator's NN-th
  * slot) is neither read nor written, as in the reference.  The time
- * argument of the reference is unused there and omitted here.  FP64 only
- * (CRS_MODE_MIXED returns CRS_ERR_UNSUPPORTED).  Two kernels per call: the
- * fused wdot kernel writes the species rates into dy, then an energy
- * kernel overwrites dy in place with the derivatives.  dy therefore must
+ * argument of the reference is unused there and omitted here.  Two kernels
+ * per call: the fused wdot kernel of the mode writes the species rates into
+ * dy, then an FP64 energy kernel overwrites dy in place with the
+ * derivatives.  mode CRS_MODE_FP64, or CRS_MODE_MIXED where the library has
+ * it (crs_get_mech_info modes; the rates as crs_eval_wdot computes them in
+ * that mode, CRS_ERR_UNSUPPORTED otherwise).  dy therefore must
  * share no entry with y or P (they may share an allocation, e.g. side by
  * side within rows); an overlap returns CRS_ERR_ARG before any launch.
  */

@@ -144,9 +144,11 @@ void mechanism_library(const crs_mech_info &info)
                                   nullptr, nullptr, 0, 0, CRS_MODE_FP64,
                                   nullptr), CRS_OK, nullptr);
 
-    MECH(crs_dydt(p, p, p, 4, 4, CRS_MODE_MIXED, nullptr),
-         CRS_ERR_UNSUPPORTED, "crs_dydt: mode 1 not available");
-    MECH(crs_dydt(p, p, p, 4, 4, 9, nullptr), CRS_ERR_ARG, "mode 9");
+    MECH(crs_dydt(nullptr, nullptr, nullptr, 0, 0, CRS_MODE_MIXED, nullptr),
+         (info.modes & (1 << CRS_MODE_MIXED)) ? CRS_OK : CRS_ERR_UNSUPPORTED,
+         "crs_dydt: mode 1 not compiled");
+    MECH(crs_dydt(p, p, p, 4, 4, 9, nullptr), CRS_ERR_ARG,
+         "crs_dydt: unknown mode 9");
     MECH(crs_dydt(p, nullptr, p, 4, 4, CRS_MODE_FP64, nullptr), CRS_ERR_ARG,
          "y is NULL");
     MECH(crs_dydt(nullptr, nullptr, nullptr, 0, 0, CRS_MODE_FP64, nullptr),

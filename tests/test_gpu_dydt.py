@@ -10,7 +10,8 @@ create_rate_subs_amd.metrics carried through the scalings:
   sum_k |h_k W_k| S_k / (rho cp_avg).
 
 mech13co is mech13 with CO moved last, so the implied species reacts and
-its term of the energy equation is non-zero.
+its term of the energy equation is non-zero.  CRS_MODE_MIXED takes the
+rates from the mixed kernel and is held to the mixed kernel's 1e-6.
 """
 
 import numpy as np
@@ -93,14 +94,33 @@ def test_dydt_integrator_layout_and_ragged_stride():
     assert (dyh[:, n:] == 5.0).all()
 
 
-def test_dydt_rejects_mixed_mode_and_bad_sizes():
+@pytest.mark.parametrize('name', ['mech13', 'mech53', 'mech111',
+                                  'mech21pc'])
+def test_mixed_dydt_within_the_mixed_bound(name):
+    """CRS_MODE_MIXED: the mixed kernel's rates, then the same FP64 energy
+    stage; against the oracle with the scales above, within the mixed
+    kernel's 1e-6 target."""
+    n = 4096
+    T, P, Y, y = _inputs(name, n, 61)
+    ref = oracle(name).dydt(P, y)
+    lib = hip_lib(name)
+    assert lib.modes & 2
+    Pd, yd = to_dev(P, y)
+    got = to_host(lib.dydt(Pd, yd, mode='mixed'))
+    f64 = to_host(lib.dydt(Pd, yd))
+    scale = _scales(name, T, P, Y)
+    e_T = _score(got[:1], ref[:1], scale[:1])
+    e_Y = _score(got[1:], ref[1:], scale[1:])
+    print(name, 'mixed dT/dt', e_T, 'dY/dt', e_Y)
+    assert e_T <= 1e-6 and e_Y <= 1e-6
+    assert not np.array_equal(got, f64)       # the mixed kernel ran
+
+
+def test_dydt_bad_sizes_and_empty():
     import torch
-    from create_rate_subs_amd.runtime import CrsError
     lib = hip_lib('mech13')
     P = torch.ones(4, dtype=torch.float64, device='cuda') * 101325.0
     y = torch.ones((lib.nsp, 4), dtype=torch.float64, device='cuda')
-    with pytest.raises(CrsError):
-        lib.dydt(P, y, mode='mixed')
     with pytest.raises(ValueError):
         lib.dydt(P, y[:lib.nsp - 1])
     e = torch.empty(0, dtype=torch.float64, device='cuda')
