@@ -397,7 +397,12 @@ def surrogate_section(args, torch, lib, specs, reacs):
         frac_normwise_le_target=(nw_raw <= args.sur_target).double().mean().item(),
         frac_normwise_le_0p1=(nw_raw <= 0.1).double().mean().item())
     out['test'] = dict(
-        states=n, gated_states_per_s=n / (gated_ms * 1e-3),
+        states=n, gate_admits_none=rmax < 0,
+        gated_path=('the calibrated gate admits no state (rmax < 0): the '
+                    'gated call skips the network and evaluates every '
+                    'state exactly' if rmax < 0 else
+                    'network, gate and exact fallback in one call'),
+        gated_states_per_s=n / (gated_ms * 1e-3),
         surrogate_only_states_per_s=n / (pred_ms * 1e-3),
         fp64_states_per_s=n / (f64_ms * 1e-3),
         speedup_vs_fp64=f64_ms / gated_ms,

@@ -880,6 +880,19 @@ __global__ void k_zero_i32(int32_t *__restrict__ p)
         *p = 0;
 }
 
+// Every state to the exact fallback: flags 1 (when given), idx[t] = t,
+// *count = n.  The gated call's path for a gate that admits no state.
+__global__ void k_all_fallback(int32_t *__restrict__ flags,
+                               int32_t *__restrict__ idx,
+                               int32_t *__restrict__ count, int32_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    idx[t] = (int32_t)t;
+    if (flags) flags[t] = 1;
+    if (t == 0) *count = n;
+}
+
 int check_gemm(const char *fn, int64_t M, int N, int64_t R)
 {
     if (M < 0 || N <= 0 || R <= 0 || N % TN != 0 || R % TR != 0)
@@ -1186,13 +1199,26 @@ int gated(const char *fn, const crs_mlp_net *net, const double *T,
         return CRS_OK;
     if (!fb_idx || !fb_count)
         return crs::fail(CRS_ERR_ARG, "%s: NULL fallback list", fn);
-    k_zero_i32<<<1, 64, 0, s>>>(fb_count);
-    CRS_TRY(crs::launched(fn));
-    CRS_TRY(launch_infer(fn, T, P, Y, n, ld, net->nsp, net->kin, net->hidden,
-                         net->layers, net->kout, net->tiles, net->bias,
-                         net->mu, net->sd, net->scale, net->ymu, net->ysd, W,
-                         net->gate, net->elem, net->n_elem, net->mw, flags,
-                         false, fb_idx, fb_count, s));
+    if (!(net->gate.rmax >= 0.0)) {
+        // a gate that admits no state (rmax < 0: no residual threshold met
+        // the error target): the network is not evaluated, every state
+        // goes to the exact kernel
+        if (n > INT32_MAX)
+            return crs::fail(CRS_ERR_ARG, "%s: n = %lld exceeds the int32 "
+                             "fallback list", fn, (long long)n);
+        k_all_fallback<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+            flags, fb_idx, fb_count, (int32_t)n);
+        CRS_TRY(crs::launched(fn));
+    } else {
+        k_zero_i32<<<1, 64, 0, s>>>(fb_count);
+        CRS_TRY(crs::launched(fn));
+        CRS_TRY(launch_infer(fn, T, P, Y, n, ld, net->nsp, net->kin,
+                             net->hidden, net->layers, net->kout, net->tiles,
+                             net->bias, net->mu, net->sd, net->scale,
+                             net->ymu, net->ysd, W, net->gate, net->elem,
+                             net->n_elem, net->mw, flags, false, fb_idx,
+                             fb_count, s));
+    }
     const int rc = exact(T, P, Y, W, fb_idx, fb_count, n, ld, exact_mode,
                          (void *)s);
     if (rc != CRS_OK)

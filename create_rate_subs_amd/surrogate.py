@@ -139,7 +139,9 @@ def gate_threshold(err, resid, target):
     within ``target``, and rmax is the residual of its last state (a state
     passes the gate when its residual is <= rmax, so a prefix never ends
     inside a run of equal residuals).  Non-finite errors or residuals are
-    never admitted.  (0.0, 0.0) when no prefix qualifies.
+    never admitted.  (-1.0, 0.0) when no prefix qualifies: a negative rmax
+    marks a gate that admits no state, and the gated evaluation then skips
+    the network altogether.
     """
     e = torch.where(torch.isfinite(err), err, torch.full_like(err, 1e300))
     r = torch.where(torch.isfinite(resid), resid,
@@ -153,7 +155,7 @@ def gate_threshold(err, resid, target):
     ends[:-1] = rs[1:] > rs[:-1]
     ok = ((worst <= target) & ends & (rs < 1e300)).nonzero()
     if ok.numel() == 0:
-        return 0.0, 0.0
+        return -1.0, 0.0
     last = int(ok.max().item())
     return float(rs[last].item()), (last + 1) / err.numel()
 
@@ -587,6 +589,11 @@ class Surrogate(object):
                      _p(Y), _p(W), n, Y.stride(0), self.exact_fn(),
                      MODES[mode], _p(flags), _p(idx), _p(cnt), _s())
             return int(cnt.item()) if read_count else cnt
+        if not self.gate.rmax >= 0.0:
+            # the gate admits no state: exact rates, no network
+            self.mech.eval_wdot(T, P, Y, mode, out=W)
+            cnt = torch.full((1,), n, dtype=torch.int32, device=W.device)
+            return n if read_count else cnt
         self.predict(T, P, Y, W, fused=False)
         flags, idx, cnt = self.gate_states(T, P, Y, W)
         m = int(cnt.item())

@@ -28,7 +28,8 @@ extern "C" {
 typedef struct crs_mlp_gate_params {
     double t_lo, t_hi; /* training box in T [K] */
     double p_lo, p_hi; /* training box in P [Pa] */
-    double rmax;       /* max element / mass balance residual, relative */
+    double rmax;       /* max element / mass balance residual, relative;
+                          < 0: the gate admits no state */
 } crs_mlp_gate_params;
 
 const char *crs_mlp_last_error(void);
@@ -165,6 +166,10 @@ typedef struct crs_mlp_net {
  * crs_eval_wdot in mode exact_mode).  Caller-owned workspace: fb_idx [n]
  * (the fallback states, in no particular order) and *fb_count (their
  * number, zeroed by this call); flags [n] (1 = sent back) may be NULL.
+ * A gate with rmax < 0 admits no state (what calibration sets when no
+ * residual threshold meets the error target): the network is then not
+ * evaluated, fb_idx gets 0..n-1, *fb_count n, flags all 1, and W is the
+ * exact kernel's throughout.
  * This is the reference's eval_wdot chain (rate_subs.py:1845-1880) with a
  * learned evaluation in front of it; the reference has no surrogate. */
 int crs_mlp_eval_gated(const crs_mlp_net *net, const double *T,

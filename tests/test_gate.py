@@ -20,14 +20,14 @@ def test_longest_prefix_within_target():
     r = _t([0.3, 0.4, 0.1, 0.2])
     assert gate_threshold(e, r, 1e-2) == (0.2, 0.5)
     assert gate_threshold(e, r, 3e-2) == (0.4, 1.0)
-    assert gate_threshold(e, r, 1e-4) == (0.0, 0.0)
+    assert gate_threshold(e, r, 1e-4) == (-1.0, 0.0)
 
 
 def test_non_finite_never_admitted():
     e = _t([1e-3, math.nan, 1e-3])
     r = _t([0.1, 0.05, 0.2])
     # the NaN-error state ranks first, so nothing is admitted
-    assert gate_threshold(e, r, 1e-2) == (0.0, 0.0)
+    assert gate_threshold(e, r, 1e-2) == (-1.0, 0.0)
     e = _t([1e-3, 1e-3, 1e-3])
     r = _t([0.1, math.inf, 0.2])
     assert gate_threshold(e, r, 1e-2) == (0.2, 2 / 3)

@@ -280,6 +280,33 @@ def test_fused_gate_flags_match_gate_kernel(name):
         print('rmax', rmax, 'fallback', int(c1.item()))
 
 
+@pytest.mark.parametrize('name', MECHS)
+def test_gate_that_admits_nothing_skips_the_network(name):
+    """rmax < 0, what calibration returns when no residual threshold meets
+    the target: the gated call (the C ABI's fused path and the Python path)
+    gives the exact FP64 rates for every state, with every state on the
+    fallback list."""
+    n = 3000
+    net, Td, Pd, Yd = _fit_random(name, n, 79)
+    assert net.fused_ok
+    net.gate.rmax = -1.0
+    exact = net.mech.eval_wdot(Td, Pd, Yd, 'fp64')
+    for fused in (True, False):
+        W = torch.full((net.nsp, n), float('nan'), dtype=torch.float64,
+                       device='cuda')
+        assert net.evaluate(Td, Pd, Yd, W, fused=fused) == n
+        assert torch.equal(W, exact)
+        if fused:
+            idx = net._bufs['fb_idx'][:n]
+            flags = net._bufs['flags'][:n]
+            assert torch.equal(idx.long().cpu(), torch.arange(n))
+            assert bool((flags == 1).all())
+    # a gate that admits states still runs the network
+    net.gate.rmax = 1e300
+    W = torch.empty_like(exact)
+    assert net.evaluate(Td, Pd, Yd, W) < n
+
+
 def test_trained_network_gain():
     """What training adds, measured on held-out states.
 
