@@ -267,6 +267,9 @@ def test_split_dydt_is_the_default():
     e = dydt.index('k_dydt_energy<<<')
     assert w < e
     assert 'check_disjoint(fn, y, NSP, "y", dy, NSP, "dy"' in dydt
+    # CRS_MODE_MIXED takes the mixed kernel's rates, the same energy stage
+    assert 'check_mode(fn, mode)' in dydt
+    assert dydt.index('k_wdot_mixed<<<') < dydt.index('k_dydt_energy<<<')
     rk4 = text[text.index('int crs_rk4('):]
     rk4 = rk4[:rk4.index('\n}\n')]
     assert rk4.index('k_wdot_f64<<<') < rk4.index('k_rk4_stage<1><<<')
