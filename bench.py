@@ -415,6 +415,36 @@ def surrogate_section(args, torch, lib, specs, reacs):
     return out
 
 
+def integrate_nonstiff_section(torch, n=1 << 20, steps=20, h=1.0e-5):
+    """The same RK4 harness with a step that moves the states: mech9ns (9
+    species / 14 reactions, a synthetic mechanism generated non-stiff, every
+    Jacobian eigenvalue below ~2e4 1/s at 1200-1500 K and 1 atm), n states
+    at 1200-1500 K and 1 atm, `steps` steps of h seconds from one captured
+    HIP graph.  That the harness converges at fourth order on this
+    mechanism is a test of its own; here the throughput and how far the
+    temperatures moved are reported."""
+    from create_rate_subs_amd import build, integrate, runtime, states
+    lib = runtime.load(build.lib_path('mech9ns'))
+    nsp = lib.nsp
+    T, _P, Y = states.random_states(nsp, n, 3101, t_range=(1200.0, 1500.0))
+    P = torch.full((n,), 101325.0, dtype=torch.float64, device='cuda')
+    y = torch.empty((nsp, n), dtype=torch.float64, device='cuda')
+    y[0] = torch.from_numpy(T).cuda()
+    y[1:] = torch.from_numpy(Y[:nsp - 1]).cuda()
+    T0 = y[0].clone()
+    out = integrate.integrate(lib, P, y, 0.0, steps * h, h)
+    dT = (y[0] - T0).abs()
+    return dict(workload='mech9ns (9 sp / 14 rxn, non-stiff), {} states at '
+                         '1200-1500 K and 1 atm, {} RK4 steps of {:g} s from '
+                         'one graph'.format(n, out['steps'], h),
+                steps=out['steps'], h=h, ms_per_step=out['ms_per_step'],
+                state_steps_per_s=n * out['steps'] / (out['ms_total'] * 1e-3),
+                max_abs_dT_K=dT.max().item(),
+                median_abs_dT_K=dT.median().item(),
+                finite_fraction=float(torch.isfinite(y).all(0).double()
+                                      .mean()))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -516,6 +546,10 @@ def main():
     line['integrate'] = integrate_section(args, torch, lib, specs, n)
     log('integrate: {:.3f} ms per RK4 step'.format(
         line['integrate']['ms_per_step']))
+    line['integrate_nonstiff'] = integrate_nonstiff_section(torch)
+    log('integrate (non-stiff): {:.3f} ms per RK4 step, max |dT| {:.1f} K'
+        .format(line['integrate_nonstiff']['ms_per_step'],
+                line['integrate_nonstiff']['max_abs_dT_K']))
     del Td, Pd, Yd, Wd
 
     if not args.no_mixed and (lib.modes & 2):
