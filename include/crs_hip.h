@@ -40,9 +40,9 @@ extern "C" {
  * Arrhenius and falloff exponentials to 1.0e-8 relative (a degree-4 FP64
  * polynomial), everything else FP64; max conditioned error 1.9e-7 against
  * CRS_MODE_FP64 (53 species / 325 reactions, 2^22 random states, tau 0.05),
- * 1.09x its throughput.  A library emitted with mixed_arith='fp32' takes
+ * 1.09-1.15x its throughput.  A library emitted with mixed_arith='fp32' takes
  * FP32 exponential cores, falloff and third-body terms here instead:
- * 1.3x, max 3.8e-6. */
+ * 1.28-1.36x, max 3.8e-6. */
 #define CRS_MODE_MIXED 1
 
 typedef struct crs_mech_info {
