@@ -225,12 +225,12 @@ def dydt_section(args, torch, lib, T, P, Y):
     out = dict(workload='w1_53 states in the reference dydt layout '
                         '(row 0 = T, rows 1..NSP-1 = Y)',
                states_per_s=n / (ms * 1e-3), kernel_ms=kern,
-               kernel='k_wdot_f64 + k_dydt_energy')
+               kernel='k_wdot_dydt + k_dydt_energy')
     if lib.modes & 2:
         mx_ms, mx_kern = timed(
             torch, lambda: lib.dydt(P, y, out=dy, mode='mixed'), steps, 1)
         out['mixed'] = dict(
-            kernel='k_wdot_mixed + k_dydt_energy', kernel_ms=mx_kern,
+            kernel='k_wdot_mixed_dydt + k_dydt_energy', kernel_ms=mx_kern,
             states_per_s=n / (mx_ms * 1e-3), speedup_vs_fp64=kern / mx_kern,
             parity='<= 1e-6 conditioned against the oracle '
                    '(tests/test_gpu_dydt.py)')

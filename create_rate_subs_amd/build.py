@@ -134,6 +134,8 @@ def device_usage(src, extra_flags=()):
 # the fused kernels (emitter name -> symbol tag) and the reaction-group
 # sizes build_with_groups tries for them
 FUSED_KERNELS = (('f64', 'k_wdot_f64E'), ('mixed', 'k_wdot_mixedE'),
+                 ('f64dydt', 'k_wdot_dydtE'),
+                 ('mixeddydt', 'k_wdot_mixed_dydtE'),
                  ('dydt', 'k_dydt_f64E'))   # the last with dydt='fused'
 GROUP_CANDIDATES = (32,) + tuple(range(36, 61)) + (64,)
 _DEFAULT_GROUP = 32
@@ -170,7 +172,7 @@ def build_with_groups(write, out_dir, extra_flags=(), verbose=False,
                       candidates=GROUP_CANDIDATES):
     """Emit and compile a mechanism library, choosing per fused kernel the
     number of reactions per basic block (emitter options split_f64,
-    split_mixed, split_dydt).
+    split_mixed, split_f64dydt, split_mixeddydt, split_dydt).
 
     ``write(opts, out_dir)`` emits the source with extra emitter options
     and returns its path.  The library is first built with 32 reactions

@@ -1589,29 +1589,29 @@ def _dydt_finish_lines(ir, K, tmids, k, wk, store='W[{r} * ld + i] = {v};'):
 
 def _dydt_energy_lines(ir, K, chunk=8):
     """Body of the energy stage of the split CONP dydt (rate_subs.py:
-    1829-1946 after eval_spec_rates): from the state (Y: row 0 = T, rows
-    1..NSP-1 the mass fractions), P and the species rates W (rows
-    0..NSP-1, from the wdot kernel), dT/dt and dY_k/dt through the output
-    object: ``out.fetch(r)`` for the rows of a chunk of ``chunk`` species,
-    then ``out.put(r, value)`` for each (rows 1..NSP-1, row 0 last).
+    1829-1946 after eval_spec_rates).  In: the state Y (row 0 = T, rows
+    1..NSP-1 the mass fractions) and S, the species rates as the wdot
+    kernel stores them for the dydt (option dydt_rates): row k + 1 holds
+    w_k W_k / rho = dY_k/dt for k < NSP-1, row 0 the bath gas's
+    w_N W_N / rho.  Out, through the output object (``out.fetch(r)`` before
+    ``out.put(r, value)``): rows 1..NSP-1 get dY_k/dt, row 0 last gets
+    dT/dt = -sum_k w_k h_k W_k / (rho cp_avg) = -sum_k h_k S_k / cp_avg
+    (h_k per unit mass; rho cancels, so neither rho nor P is needed).
 
     Two passes keep the live set small (the kernel is memory bound and wants
-    occupancy): the mass fractions are summed as they load (y_N, mw_avg,
-    cp_avg, in the reference's order), then the rates are taken in chunks
-    from the last species down, every rate of a chunk loaded before the
-    chunk's outputs: rows are written only after they have been read, so W
-    may be the output array itself."""
+    occupancy): cp_avg over the mass fractions as they load (the
+    reference's order, the bath gas's share from y_N last), then the S rows
+    in chunks of ``chunk`` species, every row of a chunk loaded before its
+    outputs, the energy sum in species order."""
     nsp = ir.nsp
     tmids = []
     for tm, _lo, _hi in ir.nasa:
         if tm not in tmids:
             tmids.append(tm)
-    ru = _lit(emitted(fmt_e8(chem.RU)))
     # rows are reached through a uniform row base and the lane's 32-bit
     # byte offset io (crs::ld_row): no 64-bit address per row to keep live
     lines = ['const uint32_t io = (uint32_t)i * 8u;',
              'const double T = crs::ld_row(Y, io);',
-             'const double P = crs::ld_row(P_, io);',
              'const double T2 = T * T;', 'const double T3 = T2 * T;',
              'const double T4 = T3 * T;', 'const double T5 = T4 * T;']
     vals = {_B1: '1.0', _BT: 'T', _BT2: 'T2', _BT3: 'T3', _BT4: 'T4',
@@ -1624,46 +1624,44 @@ def _dydt_energy_lines(ir, K, chunk=8):
                 g=gi, b=bi, v=v))
             lines.append('const double ph{g}_{b} = lo{g} ? 0.0 : {v};'.format(
                 g=gi, b=bi, v=v))
-    # pass 1: sums over the mass fractions, each term in the reference's
-    # order (eval_conc's y_N and mw_avg, then the cp_avg sum of dydt)
-    lines += ['double ysum = 0.0;', 'double mwsum = 0.0;',
-              'double cp_avg = 0.0;']
-    cps = []
-    for k in range(nsp):
-        cps.append(_nasa_masked(ir, K, k, tmids, lambda a: a[0:5],
-                                (_B1, _BT, _BT2, _BT3, _BT4)))
+    # pass 1: cp_avg over the mass fractions (y_N = 1 - their sum)
+    lines += ['double ysum = 0.0;', 'double cp_avg = 0.0;']
+    cps = [_nasa_masked(ir, K, k, tmids, lambda a: a[0:5],
+                        (_B1, _BT, _BT2, _BT3, _BT4)) for k in range(nsp)]
     for k in range(nsp - 1):
         lines.append('{{ const double yk = crs::ld_row(Y + {r} * ld, io);'
                      .format(r=k + 1))
         lines.append('  ysum = {}yk;'.format('ysum + ' if k else ''))
-        lines.append('  mwsum = {}(yk * {});'.format('mwsum + ' if k else '',
-                                                      K(ir.inv_mw[k])))
         lines.append('  cp_avg = cp_avg + {} * ({}) * yk; }}'.format(
             K(ir.ru_mw[k]), cps[k]))
     lines += ['const double y_N = 1.0 - ysum;',
-              'const double mw_avg = 1.0 / (mwsum + (y_N * {}));'.format(
-                  K(ir.inv_mw[nsp - 1])),
-              'const double rho = P * mw_avg / ({} * T);'.format(ru),
               'cp_avg = cp_avg + {} * ({}) * y_N;'.format(
                   K(ir.ru_mw[nsp - 1]), cps[nsp - 1]),
-              'const double irho = 1.0 / rho;', 'double esum = 0.0;']
-    # pass 2: the rates in chunks from the last species down
-    starts = list(range(0, nsp, chunk))[::-1]
-    for k0 in starts:
+              'double esum = 0.0;']
+    # pass 2: the scaled rates in species order, h_k S_k summed
+    # (h_k = RU/W_k (a5 + a0 T + a1 T^2/2 + ... + a4 T^5/5),
+    # rate_subs.py:1495-1565)
+    row = [k + 1 if k < nsp - 1 else 0 for k in range(nsp)]
+    for k0 in range(0, nsp, chunk):
         ks = list(range(k0, min(k0 + chunk, nsp)))
         lines.append('{')
-        lines += ['  const double w{k} = crs::ld_row(W + {k} * ld, io);'
-                  .format(k=k) for k in ks]
-        lines += ['  out.fetch({});'.format(k + 1) for k in ks
-                  if k < nsp - 1]
-        if k0 == 0:
-            lines.append('  out.fetch(0);')
-        for k in ks[::-1]:
-            lines += ['  ' + l for l in _dydt_finish_lines(
-                ir, K, tmids, k, 'w{}'.format(k), store='out.put({r}, {v});')]
+        lines += ['  const double s{k} = crs::ld_row(S + {r} * ld, io);'
+                  .format(k=k, r=row[k]) for k in ks]
+        lines += ['  out.fetch({});'.format(row[k]) for k in ks
+                  if row[k] > 0]
+        for k in ks:
+            poly = _nasa_masked(
+                ir, K, k, tmids,
+                lambda a: [a[5], a[0], a[1] / 2.0, a[2] / 3.0, a[3] / 4.0,
+                           a[4] / 5.0],
+                (_B1, _BT, _BT2, _BT3, _BT4, _BT5))
+            lines.append('  esum = esum + s{k} * ({ru} * ({p}));'.format(
+                k=k, ru=K(ir.ru_mw[k]), p=poly))
+            if row[k] > 0:
+                lines.append('  out.put({r}, s{k});'.format(r=row[k], k=k))
         lines.append('}')
     # energy equation, rate_subs.py:1910-1925
-    lines.append('out.put(0, (-1.0 / (rho * cp_avg)) * esum);')
+    lines += ['out.fetch(0);', 'out.put(0, (-1.0 / cp_avg) * esum);']
     return lines
 
 
@@ -1702,7 +1700,12 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                   for k in range(nsp - 1)]
     # conc_late: the T-only work (basis, G_k) runs while the mass fractions
     # are still in flight; eval_conc's sums wait for them afterwards
-    conc = _conc_lines(ir, K) + [_CONC_DONE]
+    # dydt_rates: each species' rate is stored as the CONP dydt's
+    # w_k W_k / rho, species k in row k + 1 and the bath gas in row 0
+    rates = bool(opts.get('dydt_rates'))
+    conc = (_conc_lines(ir, K) +
+            (['const double irho_r = 1.0 / rho;'] if rates else []) +
+            [_CONC_DONE])
     late = opts.get('conc_late', False)
     if not late:
         lines += conc
@@ -1826,8 +1829,17 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
             if dydt:
                 if k < nsp - 1:
                     lines.append('W[{r} * ld + i] = 0.0;'.format(r=k + 1))
+            elif rates:
+                lines.append('W[{r} * ld + i] = 0.0;'.format(
+                    r=k + 1 if k < nsp - 1 else 0))
             else:
                 lines.append('W[{k} * ld + i] = 0.0;'.format(k=k))
+
+    def w_store(k):
+        if rates:
+            return 'W[{r} * ld + i] = (w[{k}]) * ({mw} * irho_r);'.format(
+                r=k + 1 if k < nsp - 1 else 0, k=k, mw=K(ir.mw[k]))
+        return 'W[{k} * ld + i] = w[{k}];'.format(k=k)
     for pos, x in enumerate(rxns):
         if x.plog:
             inner, xf = _plog_fast_lines(x, K, expfn), None
@@ -1852,8 +1864,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
                 if last.get(k) == x.index:
                     lines += (_dydt_finish_lines(ir, K, tmids, k,
                                                  'w[{}]'.format(k))
-                              if dydt else
-                              ['W[{k} * ld + i] = w[{k}];'.format(k=k)])
+                              if dydt else [w_store(k)])
             continue
         inner.append('const double f = {}kf;'.format(_conc_prod(x.reac)))
         if x.rev:
@@ -1913,8 +1924,7 @@ def _fused_fast_lines(ir, K, opts, mixed=False, dydt=False, ktab=None,
             if last.get(k) == x.index:
                 lines += (_dydt_finish_lines(ir, K, tmids, k,
                                              'w[{}]'.format(k))
-                          if dydt else
-                          ['W[{k} * ld + i] = w[{k}];'.format(k=k)])
+                          if dydt else [w_store(k)])
         if opts.get('sched_barrier'):
             lines.append('__builtin_amdgcn_sched_barrier(0);')
     if mixed and not mixed_pm32:
@@ -2168,6 +2178,9 @@ def emit_hip_source(ir, opts=None):
     dydt_split = opts.get('dydt', 'split') == 'split'
     if opts.get('dydt', 'split') not in ('split', 'fused'):
         raise ValueError("dydt must be 'split' or 'fused'")
+    if dydt_split and opts['fused'] != 'fast':
+        raise ValueError("dydt='split' needs fused='fast' (its rate kernel "
+                         "is that body storing dydt-scaled rates)")
     if dydt_split:
         dopts = opts
     K = _Consts(opts['consts'])
@@ -2308,34 +2321,64 @@ def emit_hip_source(ir, opts=None):
                      '        crs_nasa_lds[q] = crs_nasa_tab[q];\n'
                      '    __syncthreads();\n'.format(n=len(vals)))
 
-    def fused_state(name, o, **kw):
+    def fused_state(name, o, tab=None, **kw):
         """Lines of one fused per-state body and the fill code of its
         kernel; with consts_rx='lds' (every fused kernel) or 'lds_fp64'
         (the FP64 ones) the body's reaction-block literals live in an LDS
-        table (declared here, filled by the kernel)."""
+        table (declared here, filled by the kernel; named after ``tab``,
+        by default ``name``, which also picks the group size)."""
         mode = opts.get('consts_rx')
+        tab = tab or name
         nfill = nasa_fill if o.get('nasa') == 'lds' else ''
         # reactions per basic block, per kernel (split_f64, split_mixed,
-        # split_dydt) over the common split_blocks
+        # split_f64dydt, split_mixeddydt, split_dydt) over the common
+        # split_blocks
         if 'split_' + name in o:
             o = dict(o, split_blocks=o['split_' + name])
         if mode not in ('lds', 'lds_fp64') or (mode == 'lds_fp64' and
                                                 name == 'mixed'):
             return _fused_fast_lines(ir, K, o, **kw), nfill
         tab = []
-        lines = _fused_fast_lines(ir, K, o, ktab=tab,
-                                  ktab_name='crs_kt_' + name, **kw)
-        n = max(len(tab), 1)
+        vals = []
+        lines = _fused_fast_lines(ir, K, o, ktab=vals,
+                                  ktab_name='crs_kt_' + tab, **kw)
+        n = max(len(vals), 1)
         body.append('__device__ const double crs_kt_{}_src[{}] = {{\n'.format(
-            name, n) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
-                                             else ' ')
-                               for q, v in enumerate(tab or [0.0])) + '};\n')
+            tab, n) + ''.join('{},{}'.format(_lit(v), '\n' if q % 8 == 7
+                                            else ' ')
+                              for q, v in enumerate(vals or [0.0])) + '};\n')
         body.append('__shared__ __align__(16) double crs_kt_{}[{}];\n\n'.format(
-            name, n))
+            tab, n))
         fill = ('    for (int q = threadIdx.x; q < {n}; q += blockDim.x)\n'
                 '        crs_kt_{name}[q] = crs_kt_{name}_src[q];\n'
-                '    __syncthreads();\n'.format(n=n, name=name))
+                '    __syncthreads();\n'.format(n=n, name=tab))
         return lines, nfill + fill
+
+    def fused_kernel(state, kernel, lines, fill, note=''):
+        """A fused per-state body as ``state`` and its batch/subset kernel
+        ``kernel`` (the signature every fused kernel shares)."""
+        body.append(
+            note + '__device__ __forceinline__ void ' + state + '(\n'
+            '    const double *__restrict__ K,\n'
+            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Y, double *__restrict__ W,\n'
+            '    int64_t ld, int64_t i' + pre_arg + ')\n{\n' +
+            _indent(lines, 4) + '}\n\n')
+        body.append(
+            '__global__ __launch_bounds__(kWdotBlock' + lb + ') void ' +
+            kernel + '(\n'
+            '    const double *__restrict__ K,\n'
+            '    const double *__restrict__ T_, const double *__restrict__ P_,\n'
+            '    const double *__restrict__ Y, double *__restrict__ W,\n'
+            '    const int32_t *__restrict__ idx, int64_t m,\n'
+            '    const int32_t *__restrict__ m_dev, int64_t ld)\n{\n' +
+            _idx_kernel_tail(fill, state + '(K, T_, P_, Y, W, ld, i{pre});',
+                             preload))
+
+    rates_note = (
+        '// The same body storing each species\' rate as the CONP dydt\'s\n'
+        '// w_k W_k / rho, species k in row k + 1 and the bath gas in row\n'
+        '// 0: the rates crs_dydt and crs_rk4 hand to their energy stage.\n')
 
     # ---- fused FP64 per-state body ------------------------------------
     # preload needs a fill to overlap with: the NASA table's
@@ -2366,26 +2409,33 @@ def emit_hip_source(ir, opts=None):
         _idx_kernel_tail(f64_fill,
                          'wdot_f64_state(K, T_, P_, Y, W, ld, i{pre});',
                          preload))
+    if dydt_split:
+        r_lines, r_fill = fused_state('f64dydt', dict(
+            opts, preload=preload, block_expr='kWdotBlock', dydt_rates=True),
+            tab='f64_dydt')
+        fused_kernel('wdot_dydt_state', 'k_wdot_dydt', r_lines, r_fill,
+                     rates_note)
 
     # ---- constant-pressure dydt (FP64), the caller of the path -------
     if dydt_split:
         body.append(
-            '// Energy stage of the CONP dydt: the state (row 0 = T, rows\n'
-            '// 1..NSP-1 the mass fractions), P and the species rates W of\n'
-            '// the wdot kernel give dT/dt and dY_k/dt, handed to the output\n'
-            '// object (fetch(row) ahead of a chunk, put(row, value)).  No\n'
-            '// contraction: the values are the same bits in k_dydt_energy\n'
-            '// and in k_rk4_stage.\n'
+            '// Energy stage of the CONP dydt: the state Y (row 0 = T, rows\n'
+            '// 1..NSP-1 the mass fractions) and the rates S the wdot kernel\n'
+            '// stored scaled for the dydt (row k + 1 = dY_k/dt, row 0 the\n'
+            '// bath gas\'s w W / rho) give dT/dt and dY_k/dt, handed to the\n'
+            '// output object (fetch(row) ahead of a chunk, put(row, value)).\n'
+            '// No contraction: the values are the same bits in\n'
+            '// k_dydt_energy and in k_rk4_stage.\n'
             'template <class Out>\n'
             '__device__ __forceinline__ void dydt_energy_state(\n'
-            '    const double *__restrict__ P_, const double *Y,\n'
-            '    const double *W, int64_t ld, int64_t i, Out &out)\n{\n'
+            '    const double *Y, const double *S, int64_t ld, int64_t i,\n'
+            '    Out &out)\n{\n'
             '#pragma clang fp contract(off)\n' +
             _indent(_dydt_energy_lines(ir, K), 4) + '}\n\n')
         body.append(
-            '// crs_dydt\'s output: the rates W were written into dy itself by\n'
-            '// the wdot kernel, and each row is overwritten only after it\n'
-            '// has been read.\n'
+            '// crs_dydt\'s output: the wdot kernel wrote dY_k/dt into dy\n'
+            '// rows 1..NSP-1 already; only row 0 changes, from the bath\n'
+            '// gas\'s scaled rate (read first) to dT/dt.\n'
             'struct DydtOut {\n'
             '    double *dy;\n'
             '    int64_t ld;\n'
@@ -2393,20 +2443,21 @@ def emit_hip_source(ir, opts=None):
             '    __device__ __forceinline__ void fetch(int) {}\n'
             '    __device__ __forceinline__ void put(int r, double v)\n'
             '    {\n'
-            '        crs::st_row(dy + r * ld, io, v);\n'
+            '        if (r == 0) crs::st_row(dy, io, v);\n'
             '    }\n'
             '};\n\n'
             '__global__ __launch_bounds__(crs::kBlock) void k_dydt_energy(\n'
-            '    const double *__restrict__ P_, const double *__restrict__ Y,\n'
-            '    double *W, int64_t n, int64_t ld)\n{\n'
+            '    const double *__restrict__ Y, double *S, int64_t n,\n'
+            '    int64_t ld)\n{\n'
             '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
             'threadIdx.x;\n'
             '    if (i >= n) return;\n'
-            '    DydtOut out{W, ld, (uint32_t)i * 8u};\n'
-            '    dydt_energy_state(P_, Y, W, ld, i, out);\n}\n\n')
+            '    DydtOut out{S, ld, (uint32_t)i * 8u};\n'
+            '    dydt_energy_state(Y, S, ld, i, out);\n}\n\n')
         body.append(
-            '// One RK4 stage: k = dydt(ys) from the rates the wdot kernel\n'
-            '// wrote for the stage state ys (the energy stage in registers),\n'
+            '// One RK4 stage: k = dydt(ys) from the scaled rates the wdot\n'
+            '// kernel wrote for the stage state ys (the energy stage in\n'
+            '// registers),\n'
             '// then, without contraction, the textbook combination:\n'
             '//   1: acc = k,      ytmp = y + h/2 k\n'
             '//   2: acc += 2 k,   ytmp = y + h/2 k\n'
@@ -2455,15 +2506,15 @@ def emit_hip_source(ir, opts=None):
             '// (at least four waves per SIMD: the kernel is memory bound)\n'
             'template <int STAGE>\n'
             '__global__ __launch_bounds__(crs::kBlock, 4) void k_rk4_stage(\n'
-            '    const double *__restrict__ P_, const double *wk, double *y,\n'
-            '    double *ytmp, double *acc, int64_t n, int64_t ld, double h,\n'
-            '    double t_step, const double *__restrict__ T0,\n'
-            '    double *__restrict__ t_ign, double dT_ign)\n{\n'
+            '    const double *wk, double *y, double *ytmp, double *acc,\n'
+            '    int64_t n, int64_t ld, double h, double t_step,\n'
+            '    const double *__restrict__ T0, double *__restrict__ t_ign,\n'
+            '    double dT_ign)\n{\n'
             '    const int64_t i = (int64_t)blockIdx.x * blockDim.x + '
             'threadIdx.x;\n'
             '    if (i >= n) return;\n'
             '    Rk4Out<STAGE> out{y, ytmp, acc, ld, (uint32_t)i * 8u, h, {}, {}};\n'
-            '    dydt_energy_state(P_, STAGE == 1 ? y : ytmp, wk, ld, i, out);\n'
+            '    dydt_energy_state(STAGE == 1 ? y : ytmp, wk, ld, i, out);\n'
             '    if (STAGE == 4 && t_ign != nullptr && t_ign[i] != t_ign[i] &&\n'
             '        y[i] >= T0[i] + dT_ign)\n'
             '        t_ign[i] = t_step;\n}\n\n')
@@ -2562,6 +2613,13 @@ def emit_hip_source(ir, opts=None):
             _idx_kernel_tail(mixed_fill,
                              'wdot_mixed_state(K, T_, P_, Y, W, ld, i{pre});',
                              preload))
+        if dydt_split:
+            r_lines, r_fill = fused_state(
+                'mixeddydt', dict(opts, preload=preload,
+                                  block_expr='kWdotBlock', dydt_rates=True),
+                tab='mixed_dydt', mixed=True)
+            fused_kernel('wdot_mixed_dydt_state', 'k_wdot_mixed_dydt',
+                         r_lines, r_fill, rates_note)
 
     header = (
         '// Generated by create_rate_subs_amd.hip_emitter for mechanism '
@@ -2597,14 +2655,15 @@ _DYDT_RK4_SPLIT = '''int crs_dydt(const double *P, const double *y, double *dy, 
     hipStream_t s = (hipStream_t)stream;
     const double *K = nullptr;
     CRS_TRY(table(fn, &K));
-    // the species rates of the state (T = row 0, Y = rows 1..) into dy,
-    // from the mode's fused kernel, then the FP64 energy stage in place
+    // the mode's fused kernel stores the state's (T = row 0, Y = rows
+    // 1..) dY_k/dt = w_k W_k / rho into dy rows 1.. and the bath gas's
+    // w W / rho into row 0; the FP64 energy stage then writes dT/dt there
     for (int64_t c0 = 0; c0 < n; c0 += crs::kMaxChunk) {{
         const int64_t c = n - c0 < crs::kMaxChunk ? n - c0 : crs::kMaxChunk;
-@MIXED_RATES@        k_wdot_f64<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
+@MIXED_RATES@        k_wdot_dydt<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
             K, y + c0, P + c0, y + ld + c0, dy + c0, nullptr, c, nullptr, ld);
         k_dydt_energy<<<crs::grid_for(c), crs::kBlock, 0, s>>>(
-            P + c0, y + c0, dy + c0, c, ld);
+            y + c0, dy + c0, c, ld);
     }}
     return crs::launched(fn);
 }}
@@ -2655,20 +2714,20 @@ int crs_rk4(const double *P, double *y, double *work, int64_t n, int64_t ld,
     for (int step = 0; step < nsteps; ++step) {{
         for (int stage = 1; stage <= 4; ++stage) {{
             const double *ys = stage == 1 ? y : ytmp;
-            k_wdot_f64<<<wgrid, kWdotBlock, 0, s>>>(K, ys, P, ys + ld, wk,
-                                                     nullptr, n, nullptr, ld);
+            k_wdot_dydt<<<wgrid, kWdotBlock, 0, s>>>(K, ys, P, ys + ld, wk,
+                                                      nullptr, n, nullptr, ld);
             if (stage == 1)
                 k_rk4_stage<1><<<grid, crs::kBlock, 0, s>>>(
-                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+                    wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
             else if (stage == 2)
                 k_rk4_stage<2><<<grid, crs::kBlock, 0, s>>>(
-                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+                    wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
             else if (stage == 3)
                 k_rk4_stage<3><<<grid, crs::kBlock, 0, s>>>(
-                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+                    wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
             else
                 k_rk4_stage<4><<<grid, crs::kBlock, 0, s>>>(
-                    P, wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
+                    wk, y, ytmp, acc, n, ld, h, t, T0, t_ign, dT_ign);
         }}
         t += h;
     }}
@@ -2777,9 +2836,9 @@ def _abi_text(ir, mixed=True, dydt_split=True):
 '''
         mixed_ok = 'mode == CRS_MODE_MIXED'
         mixed_rates = '''        if (mode == CRS_MODE_MIXED)
-            k_wdot_mixed<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0, s>>>(
-                K, y + c0, P + c0, y + ld + c0, dy + c0, nullptr, c, nullptr,
-                ld);
+            k_wdot_mixed_dydt<<<crs::grid_for(c, kWdotBlock), kWdotBlock, 0,
+                                s>>>(K, y + c0, P + c0, y + ld + c0, dy + c0,
+                                     nullptr, c, nullptr, ld);
         else
 '''
     else:

@@ -156,9 +156,9 @@ int crs_eval_wdot_subset_dev(const double *T, const double *P,
  * dY_k/dt = wdot_k W_k / rho.  Any row beyond NSP-1 (an integrator's NN-th
  * slot) is neither read nor written, as in the reference.  The time
  * argument of the reference is unused there and omitted here.  Two kernels
- * per call: the fused wdot kernel of the mode writes the species rates into
- * dy, then an FP64 energy kernel overwrites dy in place with the
- * derivatives.  mode CRS_MODE_FP64, or CRS_MODE_MIXED where the library has
+ * per call: the fused wdot kernel of the mode writes dY_k/dt = wdot_k W_k /
+ * rho into dy rows 1..NSP-1 (and the bath gas's wdot W / rho into row 0),
+ * then an FP64 energy kernel replaces row 0 with dT/dt.  mode CRS_MODE_FP64, or CRS_MODE_MIXED where the library has
  * it (crs_get_mech_info modes; the rates as crs_eval_wdot computes them in
  * that mode, CRS_ERR_UNSUPPORTED otherwise).  dy therefore must
  * share no entry with y or P (they may share an allocation, e.g. side by

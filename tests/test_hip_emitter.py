@@ -255,43 +255,65 @@ def test_fused_kernels_use_the_transforms():
 
 
 def test_split_dydt_is_the_default():
-    """By default crs_dydt runs the wdot kernel into dy and then the energy
-    stage in place, crs_rk4 the wdot kernel and the energy + RK4-stage
-    kernel; the fused dydt kernel is not emitted."""
+    """By default crs_dydt runs the rate kernel that stores the dydt's
+    scaled rates into dy and then the energy stage, crs_rk4 the same rate
+    kernel and the energy + RK4-stage kernel; the fused dydt kernel is not
+    emitted, and the headline kernel's body is not changed by any of it."""
     text = hip_emitter.emit_hip_source(_ir('mech53'))
     assert 'void k_dydt_f64(' not in text
     assert 'void k_rk4_combine(' not in text
     assert 'void k_dydt_energy(' in text and 'void k_rk4_stage(' in text
     dydt = text[text.index('int crs_dydt('):text.index('int crs_rk4(')]
-    w = dydt.index('k_wdot_f64<<<')
+    w = dydt.index('k_wdot_dydt<<<')
     e = dydt.index('k_dydt_energy<<<')
     assert w < e
     assert 'check_disjoint(fn, y, NSP, "y", dy, NSP, "dy"' in dydt
     # CRS_MODE_MIXED takes the mixed kernel's rates, the same energy stage
     assert 'check_mode(fn, mode)' in dydt
-    assert dydt.index('k_wdot_mixed<<<') < dydt.index('k_dydt_energy<<<')
+    assert dydt.index('k_wdot_mixed_dydt<<<') < dydt.index('k_dydt_energy<<<')
     rk4 = text[text.index('int crs_rk4('):]
     rk4 = rk4[:rk4.index('\n}\n')]
-    assert rk4.index('k_wdot_f64<<<') < rk4.index('k_rk4_stage<1><<<')
+    assert rk4.index('k_wdot_dydt<<<') < rk4.index('k_rk4_stage<1><<<')
+    # the rate kernel: every rate store scaled by W_k / rho, species k in
+    # row k + 1, the bath gas (53rd) in row 0; the FP64 body otherwise the
+    # headline kernel's
+    def state(name):
+        t = text[text.index('void {}('.format(name)):]
+        return t[:t.index('\n}\n')]
+    f64, rates = state('wdot_f64_state'), state('wdot_dydt_state')
+    stores = re.findall(r'crs::st_row\(W \+ (\d+) \* crs::fresh_s<\d+>\(ld\), '
+                        r'io, (.*)\);', rates)
+    assert sorted(int(r) for r, _v in stores) == list(range(53))
+    for r, v in stores:
+        m = re.match(r'\(w\[(\d+)\]\) \* \(.* \* irho_r\)$', v)
+        assert v == '0.0' or m, v
+        if m:
+            assert int(r) == (int(m.group(1)) + 1) % 53
+    assert 'irho_r' not in f64
+    assert f64.count('crs::exp2_64k(') == rates.count('crs::exp2_64k(')
     # the energy stage is not contracted, so the RK4 kernel's stage
     # derivatives are the bits crs_dydt returns
     body = text[text.index('void dydt_energy_state('):]
     body = body[:body.index('\n}\n')]
     assert '#pragma clang fp contract(off)' in body
-    # rates in chunks from the last species down; within a chunk every
-    # rate is loaded before the first output, and a chunk writes rows k+1
-    # whose rates (row k+1) were loaded by this chunk or an earlier one
-    # (W may be dy itself)
+    assert 'P_' not in body and 'rho' not in body
+    # the scaled rates in chunks, each row loaded before its output; row
+    # k + 1 carries species k, row 0 the bath gas, written last
     loaded = set()
+    puts = []
     for line in body.split('\n'):
-        m = re.search(r'const double w(\d+) = crs::ld_row\(W \+ (\d+) \* ld', line)
+        m = re.search(r'const double s(\d+) = crs::ld_row\(S \+ (\d+) \* ld',
+                      line)
         if m:
-            assert m.group(1) == m.group(2)
-            loaded.add(int(m.group(1)))
+            k, r = int(m.group(1)), int(m.group(2))
+            assert r == (k + 1) % 53
+            loaded.add(r)
         m = re.search(r'out\.put\((\d+),', line)
         if m:
-            assert int(m.group(1)) in loaded, line
+            puts.append(int(m.group(1)))
+            assert puts[-1] in loaded, line
     assert loaded == set(range(53))
+    assert puts[-1] == 0 and sorted(puts) == list(range(53))
     fused = hip_emitter.emit_hip_source(_ir('mech53'), {'dydt': 'fused'})
     assert 'void k_dydt_f64(' in fused and 'void k_rk4_combine(' in fused
     assert 'void k_dydt_energy(' not in fused
