@@ -25,6 +25,7 @@ they were built from (opts.json).  One JSON line per (variant, mode).
 import json
 import os
 import sys
+import time
 
 from . import build, hip_emitter, workloads
 
@@ -176,6 +177,14 @@ def sweep(mech, exps=range(10, 23, 2), steps=10, seed=2001, rounds=3):
     nmax = 1 << max(exps)
     T, P, Y = states.random_states(len(specs), nmax, seed)
     Td, Pd, Yd = (torch.from_numpy(a).cuda() for a in (T, P, Y))
+    # the part's clocks ramp for tens of milliseconds after idling: warm
+    # up on the largest batch for half a second before the first size
+    warm = torch.empty_like(Yd)
+    t_end = time.perf_counter() + 0.5
+    while time.perf_counter() < t_end:
+        lib.eval_wdot(Td, Pd, Yd, 'fp64', out=warm)
+        torch.cuda.synchronize()
+    del warm
     for k in exps:
         n = 1 << k
         Tn, Pn, Yn = Td[:n], Pd[:n], Yd[:, :n].contiguous()
